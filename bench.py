@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""COLA particle-steps/s on one MI355X.
+
+Workload (default): 512^3 particles, 1024^3 PM mesh, periodic box
+1024 Mpc/h, flat LCDM (Om 0.31, Ob 0.049, h 0.67, ns 0.96, sigma8 0.81,
+Eisenstein-Hu no-wiggle P(k)), 2LPT initial conditions at a = 0.1 from
+seeded white noise, 10 COLA steps linear in a to a = 1, FP32 state with FP64
+time coefficients.
+
+A step is one pm.force + cola.kick + cola.drift over all particles, all
+device-resident.  --warmup steps run first on a copy of the initial state,
+which is then restored, so the timed --steps are steps 0..K-1 of a K-step run
+from a = 0.1.  value = Np * K / T over the timed K steps.
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" fields).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def schedule(a0, a1, nstep):
+    return [(a0 + (a1 - a0) / nstep * (i + 0.5), a0 + (a1 - a0) / nstep * (i + 1.0))
+            for i in range(nstep)]
+
+
+def cola_step(fs, p, a_vel, a_pos):
+    fs.pm.force(p)
+    fs.cola.kick(p, a_vel)
+    fs.cola.drift(p, a_pos)
+
+
+def cpu_baseline(n_p, nc, box, sub, workers):
+    """Oracle port on the host: one COLA step with the four FFTs at full mesh
+    size (FP32 scipy.fft, `workers` threads) and the particle loops (paint,
+    readout, kick, drift) over a 1/`sub` subsample, scaled back by `sub`."""
+    import numpy as np
+    from oracle import cic as ocic
+    from oracle import cola as ocola
+    from oracle import cosmology as ocosmo
+    from oracle import pm as opm
+
+    rng = np.random.default_rng(0)
+    npart = n_p**3 // sub
+    x = rng.uniform(0, box, (npart, 3)).astype(np.float32)
+    v = np.zeros_like(x)
+    dx1 = rng.standard_normal((npart, 3)).astype(np.float32)
+    dx2 = rng.standard_normal((npart, 3)).astype(np.float32)
+    cosmo = ocosmo.Cosmology(0.31)
+    pm_factor = nc / n_p
+
+    t0 = time.perf_counter()
+    delta = ocic.paint(x, nc, box, pm_factor, wdtype=np.float32).astype(np.float32)
+    t_paint = time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    meshes = opm.force_meshes_fp32(delta, box, workers=workers)
+    t_fft = time.perf_counter() - t0
+    del delta
+
+    t0 = time.perf_counter()
+    f = ocic.readout(x, meshes, box, wdtype=np.float32).astype(np.float32)
+    del meshes
+    v = ocola.kick(v, f, dx1, dx2, cosmo, 0.1, 0.1, 0.145).astype(np.float32)
+    x = ocola.drift(x, v, dx1, dx2, cosmo, 0.1, 0.145, 0.19).astype(np.float32)
+    t_part = time.perf_counter() - t0
+
+    t_step = t_fft + sub * (t_paint + t_part)
+    return {
+        "value": n_p**3 / t_step,
+        "unit": "particle-steps/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": (f"1 COLA step of the FP32 oracle port at {nc}^3 mesh: 4 FFTs at full size "
+                   f"(scipy.fft, {workers} threads, {t_fft:.1f} s) + paint/readout/kick/drift "
+                   f"over 1/{sub} of the {n_p}^3 particles (numpy, 1 thread, "
+                   f"{t_paint + t_part:.1f} s) scaled x{sub}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--np", type=int, default=512, help="particles per dimension")
+    ap.add_argument("--nc", type=int, default=1024, help="PM mesh per dimension")
+    ap.add_argument("--box", type=float, default=1024.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--cpu-sub", type=int, default=8)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "
+              f"(the multi-GPU slab decomposition is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    import fs2_amd.fs as fs
+    c = fs._fs
+
+    fs.cosmology.init(0.31)
+    ps = fs.PowerSpectrum.eisenstein_hu(0.31, 0.049, 0.67, 0.96, 0.81)
+    print(f"[bench] 2LPT {args.np}^3 box {args.box}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    p = fs.lpt.init(args.np, args.box, 0.1, ps, args.seed, "cola")
+    c._synchronize()
+    t_ic = time.perf_counter() - t0
+    fs.pm.init(args.nc, args.nc / args.np, args.box)
+    snap = fs.Particles(args.np**3, args.box)
+    snap.copy_from(p)
+
+    sched = schedule(0.1, 1.0, args.steps)
+    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        a_vel, a_pos = sched[i % len(sched)]
+        cola_step(fs, p, a_vel, a_pos)
+    c._synchronize()
+    p.copy_from(snap)
+    c._synchronize()
+
+    c._profile_reset()
+    c._profile_enable(True)
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for a_vel, a_pos in sched:
+        cola_step(fs, p, a_vel, a_pos)
+    c._synchronize()
+    T = time.perf_counter() - t0
+    prof = c._profile_get()
+    c._profile_enable(False)
+
+    npart = args.np**3
+    value = npart * args.steps / T
+    kernels = {}
+    for name, launches, total_ms, bytes_per in prof:
+        if launches == 0:
+            continue
+        avg = total_ms / launches
+        kernels[name] = {"launches": launches, "avg_ms": round(avg, 4),
+                         "total_ms": round(total_ms, 3),
+                         "GBps": round(bytes_per / (avg * 1e-3) / 1e9, 1)}
+    dom = max(prof, key=lambda r: r[2])
+    dom_avg_s = dom[2] / dom[1] * 1e-3
+    achieved = dom[3] / dom_avg_s / 1e9
+    ng = args.nc**3
+    nk = args.nc**2 * (args.nc // 2 + 1)
+    step_bytes = 156.0 * npart + 32.0 * ng + 32.0 * nk
+    out = {
+        "metric": "COLA particle-steps/s, 512^3 p/1024^3 mesh, 1 GPU",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": T / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (2LPT from seeded white noise, EH98 no-wiggle P(k))",
+        "config": {"workload": f"COLA {args.np}^3 particles, {args.nc}^3 mesh, box "
+                               f"{args.box:g} Mpc/h, {args.steps} steps a=0.1->1",
+                   "np": args.np, "nc": args.nc, "box": args.box,
+                   "paint": "atomic", "ic_seconds": round(t_ic, 3)},
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+        "step_roofline": {"algorithmic_bytes": step_bytes,
+                          "achieved_GBps": round(step_bytes / (T / args.steps) / 1e9, 1),
+                          "frac": round(step_bytes / (T / args.steps) / 1e9 / PEAK_HBM_GBS, 4)},
+        "kernels": kernels,
+        "cpu_baseline": None,
+    }
+    if args.cpu_baseline == "on":
+        del p, snap
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        try:
+            workers = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            workers = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(args.np, args.nc, args.box, args.cpu_sub, workers)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,86 @@
+"""Build recipe for libfs2hip.so (HIP, gfx950) and the fs._fs CPython module.
+
+    python -m fs2_amd.build            # build everything in place
+
+libfs2hip.so   hipcc --offload-arch=gfx950 over fs2_amd/csrc/*.hip, *.cpp
+fs/_fs.so      g++ over fs2_amd/csrc/py/py_fs.cpp, linked to libfs2hip.so
+               with an $ORIGIN rpath
+
+Objects go to fs2_amd/_build/ (git-ignored); the two .so files are written
+in place so that they travel with the source tree.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, 'csrc')
+INCLUDE = os.path.join(ROOT, 'include')
+BUILD = os.path.join(HERE, '_build')
+LIB = os.path.join(HERE, 'libfs2hip.so')
+PYEXT = os.path.join(HERE, 'fs', '_fs.so')
+
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+ARCH = 'gfx950'
+
+HIP_FLAGS = [
+    '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
+    # packed FP32 instructions are kept out of every kernel
+    '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+    '-munsafe-fp-atomics',
+    '-I', INCLUDE, '-I', CSRC,
+    '-Wall', '-Wno-unused-function', '-Wno-unused-variable',
+]
+
+SOURCES = ['fft.hip', 'pm_kernels.hip', 'cola_kernels.hip', 'lpt_kernels.hip',
+           'capi.cpp', 'host_numerics.cpp']
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError('build step failed:\n' + ' '.join(cmd) + '\n' + r.stdout)
+    return r.stdout
+
+
+def _compile(src):
+    obj = os.path.join(BUILD, os.path.basename(src) + '.o')
+    lang = ['-x', 'hip'] if src.endswith('.hip') else ['-x', 'hip']
+    _run([HIPCC] + lang + HIP_FLAGS + ['-c', os.path.join(CSRC, src), '-o', obj])
+    return obj
+
+
+def build_lib(jobs=8):
+    os.makedirs(BUILD, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIB + '.tmp'
+    _run([HIPCC, f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_pyext():
+    import numpy
+    py_inc = sysconfig.get_paths()['include']
+    src = os.path.join(CSRC, 'py', 'py_fs.cpp')
+    tmp = PYEXT + '.tmp'
+    _run(['g++', '-O2', '-std=c++17', '-fPIC', '-shared', '-Wall',
+          '-I', py_inc, '-I', numpy.get_include(), '-I', INCLUDE,
+          src, '-o', tmp, '-L', HERE, '-lfs2hip', "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, PYEXT)
+    return PYEXT
+
+
+def build_all(jobs=8):
+    build_lib(jobs)
+    build_pyext()
+
+
+if __name__ == '__main__':
+    build_all()
+    print('built', LIB, PYEXT)
+    sys.exit(0)

@@ -1,0 +1,969 @@
+// C ABI of libfs2hip: handles, error codes, device buffers and the host-side
+// orchestration of the PM / COLA / 2LPT hot path.  See include/fs2hip.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fs2_internal.h"
+
+// ---- handles ----------------------------------------------------------------------------
+struct fs2_grid {
+  int nc = 0;
+  float* d = nullptr;   // padded [nc][nc][2(nc/2+1)]
+  bool owning = true;
+};
+
+struct fs2_particles {
+  uint64_t np_alloc = 0, np_local = 0, np_total = 0;
+  double boxsize = 0.0;
+  double a_x = 0.0, a_v = 0.0, a_f = 0.0;
+  float* planes = nullptr;   // x, v, dx1, dx2, force: each [3][np_alloc]
+  uint64_t* id = nullptr;
+  bool wrap_pending = false;
+  float* field(int k) const { return planes + (size_t)k * 3 * np_alloc; }
+  float* x() const { return field(0); }
+  float* v() const { return field(1); }
+  float* dx1() const { return field(2); }
+  float* dx2() const { return field(3); }
+  float* f() const { return field(4); }
+};
+
+enum PmStatus { PM_DONE = 0, PM_DENSITY_DONE = 1, PM_FORCE_DONE = 2 };   // py/pm.h:8
+
+struct fs2_pm {
+  int nc = 0;
+  double pm_factor = 1.0;
+  double boxsize = 0.0;
+  float boxsize_f = 0.f;
+  float* mesh = nullptr;      // density, then delta_k in place
+  float* force[3] = {nullptr, nullptr, nullptr};
+  fs2_grid density;           // borrowed view of mesh
+  int status = PM_DONE;
+  int variant = FS2_PAINT_ATOMIC;
+  bool mesh_is_k = false;
+};
+
+namespace fs2 {
+double power_P(const fs2_power* ps, double k);
+fs2_power* power_load(const char* filename);
+fs2_power* power_eh98(double om, double ob, double h, double ns, double sigma8);
+double power_sigma(const fs2_power* ps, double R);
+
+// ---- errors -----------------------------------------------------------------------------
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(int code, const std::string& msg) {
+  (void)code;
+  g_err = msg;
+}
+
+void fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  throw Error{code, std::string(buf)};
+}
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return FS2_OK;
+  } catch (const Error& e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(FS2_ERR_MEMORY, "host allocation failed");
+    return FS2_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    set_error(FS2_ERR_RUNTIME, e.what());
+    return FS2_ERR_RUNTIME;
+  }
+}
+
+// ---- runtime ----------------------------------------------------------------------------
+namespace {
+std::mutex g_init_mu;
+bool g_inited = false;
+hipStream_t g_stream = nullptr;
+int g_loglevel = 2;
+}
+
+void device_init() {
+  std::lock_guard<std::mutex> lock(g_init_mu);
+  if (g_inited) return;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    fail(FS2_ERR_DEVICE, "no HIP device available (hipGetDeviceCount: %s, n=%d)",
+         hipGetErrorString(e), n);
+  FS2_HIP(hipSetDevice(0));
+  FS2_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_inited = true;
+}
+
+hipStream_t stream() {
+  if (!g_inited) device_init();
+  return g_stream;
+}
+
+void* dmalloc(size_t bytes) {
+  device_init();
+  void* p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(FS2_ERR_MEMORY, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  return p;
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+void sync() { FS2_HIP(hipStreamSynchronize(stream())); }
+
+int loglevel() { return g_loglevel; }
+
+void logf(int level, const char* fmt, ...) {
+  if (level < g_loglevel) return;
+  va_list ap;
+  va_start(ap, fmt);
+  std::vfprintf(stderr, fmt, ap);
+  va_end(ap);
+}
+
+// ---- kernel timing ----------------------------------------------------------------------------
+namespace {
+struct ProfRec {
+  std::string name;
+  uint64_t launches = 0;
+  double total_ms = 0.0;
+  double bytes = 0.0;
+};
+struct Pending {
+  int rec;
+  hipEvent_t a, b;
+};
+bool g_prof = false;
+std::vector<ProfRec> g_recs;
+std::vector<Pending> g_pending;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t get_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  FS2_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void prof_resolve() {
+  if (g_pending.empty()) return;
+  FS2_HIP(hipStreamSynchronize(stream()));
+  for (const Pending& p : g_pending) {
+    float ms = 0.f;
+    FS2_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    g_recs[p.rec].total_ms += ms;
+    g_recs[p.rec].launches += 1;
+    g_event_pool.push_back(p.a);
+    g_event_pool.push_back(p.b);
+  }
+  g_pending.clear();
+}
+}  // namespace
+
+int prof_begin(const char* name, double bytes) {
+  if (!g_prof) return -1;
+  int rec = -1;
+  for (size_t i = 0; i < g_recs.size(); ++i)
+    if (g_recs[i].name == name) rec = (int)i;
+  if (rec < 0) {
+    g_recs.push_back(ProfRec{name, 0, 0.0, bytes});
+    rec = (int)g_recs.size() - 1;
+  }
+  g_recs[rec].bytes = bytes;
+  Pending p{rec, get_event(), get_event()};
+  FS2_HIP(hipEventRecord(p.a, stream()));
+  g_pending.push_back(p);
+  if (g_pending.size() > 4096) prof_resolve();
+  return (int)g_pending.size() - 1;
+}
+
+void prof_end(int token) {
+  if (token < 0 || !g_prof) return;
+  FS2_HIP(hipEventRecord(g_pending[token].b, stream()));
+}
+
+// ---- helpers ----------------------------------------------------------------------------------
+namespace {
+
+size_t grid_floats(int nc) { return (size_t)nc * nc * (2 * (nc / 2 + 1)); }
+
+fs2_particles* check_p(fs2_particles* p) {
+  if (!p) fail(FS2_ERR_VALUE, "null particles handle");
+  return p;
+}
+
+PosArgs pos_of(fs2_particles* p) { return PosArgs{p->x(), (int64_t)p->np_alloc, (int64_t)p->np_local}; }
+
+void flush_wrap(fs2_particles* p) {
+  if (p->wrap_pending) {
+    wrap_positions(pos_of(p), (float)p->boxsize);
+    p->wrap_pending = false;
+  }
+}
+
+// Device planes [3][stride] -> host rows (n, 3)
+void download_rows(const float* d, uint64_t stride, uint64_t n, float* out) {
+  if (n == 0) return;
+  std::vector<float> tmp(3 * n);
+  for (int a = 0; a < 3; ++a)
+    FS2_HIP(hipMemcpyAsync(tmp.data() + a * n, d + a * stride, n * sizeof(float),
+                           hipMemcpyDeviceToHost, stream()));
+  sync();
+  for (uint64_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) out[3 * i + a] = tmp[a * n + i];
+}
+
+void upload_rows(float* d, uint64_t stride, uint64_t n, const float* in) {
+  if (n == 0) return;
+  std::vector<float> tmp(3 * n);
+  for (uint64_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) tmp[a * n + i] = in[3 * i + a];
+  for (int a = 0; a < 3; ++a)
+    FS2_HIP(hipMemcpyAsync(d + a * stride, tmp.data() + a * n, n * sizeof(float),
+                           hipMemcpyHostToDevice, stream()));
+  sync();
+}
+
+fs2_grid* grid_new(int nc) {
+  if (nc < 2 || !fft_supported(nc)) fail(FS2_ERR_VALUE, "grid size %d not supported", nc);
+  fs2_grid* g = new fs2_grid();
+  g->nc = nc;
+  g->d = (float*)dmalloc(grid_floats(nc) * sizeof(float));
+  FS2_HIP(hipMemsetAsync(g->d, 0, grid_floats(nc) * sizeof(float), stream()));
+  return g;
+}
+
+void grid_get_real(const fs2_grid* g, float* out) {
+  const int n = g->nc, ncz = 2 * (n / 2 + 1);
+  FS2_HIP(hipMemcpy2DAsync(out, n * sizeof(float), g->d, ncz * sizeof(float), n * sizeof(float),
+                           (size_t)n * n, hipMemcpyDeviceToHost, stream()));
+  sync();
+}
+
+void grid_set_real(fs2_grid* g, const float* in) {
+  const int n = g->nc, ncz = 2 * (n / 2 + 1);
+  FS2_HIP(hipMemcpy2DAsync(g->d, ncz * sizeof(float), in, n * sizeof(float), n * sizeof(float),
+                           (size_t)n * n, hipMemcpyHostToDevice, stream()));
+  sync();
+}
+
+float lpt_offset = 0.5f;   // py/lpt.cpp:29
+
+}  // namespace
+}  // namespace fs2
+
+using namespace fs2;
+
+extern "C" {
+
+// ---- library --------------------------------------------------------------------------------------
+const char* fs2_last_error(void) { return g_err.c_str(); }
+int fs2_version(void) { return 1; }
+int fs2_set_loglevel(int level) {
+  g_loglevel = level;
+  return FS2_OK;
+}
+int fs2_device_init(void) { return guard([] { device_init(); }); }
+int fs2_synchronize(void) { return guard([] { sync(); }); }
+int fs2_get_stream(void** s) {
+  return guard([&] { *s = (void*)stream(); });
+}
+
+// ---- cosmology ------------------------------------------------------------------------------------
+int fs2_cosmology_init(double omega_m) { return guard([&] { host::cosmology_init(omega_m); }); }
+int fs2_cosmology_D_growth(double a, double* out) {
+  return guard([&] { *out = host::D_growth(a); });
+}
+int fs2_cosmology_D2_growth(double a, double* out) {
+  return guard([&] { *out = host::D2_growth(a, host::D_growth(a)); });
+}
+int fs2_cosmology_f_growth_rate(double a, double* out) {
+  return guard([&] { *out = host::f_growth_rate(a); });
+}
+int fs2_cosmology_Dv_growth(double a, double* out) {
+  return guard([&] { *out = host::Dv_growth(a, host::D_growth(a)); });
+}
+int fs2_cosmology_hubble(double a, double* out) {
+  return guard([&] {
+    (void)host::omega_m();
+    *out = host::hubble(a);
+  });
+}
+
+int fs2_cola_kick_coefficients(double a_v, double a_x, double a_vel, double* kick_factor, double* q1,
+                               double* q2) {
+  return guard([&] {
+    // py/cola.cpp:57-66
+    const double nLPT = -2.5;
+    const double om = host::omega_m();
+    const double ai = a_v, a = a_x, af = a_vel;
+    *kick_factor = (std::pow(af, nLPT) - std::pow(ai, nLPT)) /
+                   (nLPT * std::pow(a, nLPT) * std::sqrt(om / a + (1.0 - om) * a * a));
+    const double g1 = host::D_growth(a);
+    const double g2 = host::D2_growth(a, g1);
+    *q1 = g1;
+    *q2 = g2 - g1 * g1;
+  });
+}
+
+int fs2_cola_drift_coefficients(double a_x, double a_v, double a_pos, double* dt, double* da1,
+                                double* da2) {
+  return guard([&] {
+    // py/cola.cpp:102-109
+    *dt = host::drift_Sq(a_x, a_pos, a_v);
+    const double gi = host::D_growth(a_x), gf = host::D_growth(a_pos);
+    *da1 = gf - gi;
+    *da2 = host::D2_growth(a_pos, gf) - host::D2_growth(a_x, gi);
+  });
+}
+
+// ---- power --------------------------------------------------------------------------------------------
+int fs2_power_alloc(const char* filename, fs2_power** out) {
+  return guard([&] { *out = power_load(filename); });
+}
+int fs2_power_alloc_eh98(double om, double ob, double h, double ns, double sigma8, fs2_power** out) {
+  return guard([&] { *out = power_eh98(om, ob, h, ns, sigma8); });
+}
+void fs2_power_free(fs2_power* ps) { delete ps; }
+int fs2_power_n(const fs2_power* ps, int* n) {
+  return guard([&] { *n = ps->kind == 0 ? (int)ps->logk.size() : 0; });
+}
+int fs2_power_i(const fs2_power* ps, int i, double* k, double* P) {
+  return guard([&] {
+    if (ps->kind != 0 || i < 0 || i >= (int)ps->logk.size())
+      fail(FS2_ERR_INDEX, "power spectrum index %d out of range", i);
+    *k = std::exp(ps->logk[i]);
+    *P = std::exp(ps->logP[i]);
+  });
+}
+int fs2_power_P(const fs2_power* ps, double k, double* P) {
+  return guard([&] { *P = power_P(ps, k); });
+}
+int fs2_power_sigma(const fs2_power* ps, double R, double* sigma) {
+  return guard([&] { *sigma = power_sigma(ps, R); });
+}
+
+// ---- particles ----------------------------------------------------------------------------------------
+int fs2_particles_alloc(uint64_t np_alloc, double boxsize, fs2_particles** out) {
+  return guard([&] {
+    fs2_particles* p = new fs2_particles();
+    p->np_alloc = np_alloc;
+    p->boxsize = boxsize;
+    try {
+      const size_t nf = (size_t)15 * std::max<uint64_t>(np_alloc, 1);
+      p->planes = (float*)dmalloc(nf * sizeof(float));
+      p->id = (uint64_t*)dmalloc(std::max<uint64_t>(np_alloc, 1) * sizeof(uint64_t));
+      FS2_HIP(hipMemsetAsync(p->planes, 0, nf * sizeof(float), stream()));
+      FS2_HIP(hipMemsetAsync(p->id, 0, std::max<uint64_t>(np_alloc, 1) * sizeof(uint64_t), stream()));
+    } catch (...) {
+      dfree(p->planes);
+      dfree(p->id);
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+void fs2_particles_free(fs2_particles* p) {
+  if (!p) return;
+  if (g_inited) (void)hipStreamSynchronize(g_stream);
+  dfree(p->planes);
+  dfree(p->id);
+  delete p;
+}
+
+int fs2_particles_len(const fs2_particles* p, uint64_t* n) {
+  return guard([&] { *n = p->np_local; });
+}
+int fs2_particles_np_total(const fs2_particles* p, uint64_t* n) {
+  return guard([&] { *n = p->np_total; });
+}
+int fs2_particles_update_np_total(fs2_particles* p) {
+  return guard([&] { p->np_total = p->np_local; });
+}
+int fs2_particles_np_allocated(const fs2_particles* p, uint64_t* n) {
+  return guard([&] { *n = p->np_alloc; });
+}
+int fs2_particles_boxsize(const fs2_particles* p, double* b) {
+  return guard([&] { *b = p->boxsize; });
+}
+
+int fs2_particles_append(fs2_particles* p, const double* x, uint64_t n, int64_t row_stride,
+                         int64_t col_stride) {
+  return guard([&] {
+    check_p(p);
+    if (n == 0) {
+      p->np_total = p->np_local;
+      return;
+    }
+    if (p->np_local + n > p->np_alloc)
+      fail(FS2_ERR_MEMORY, "append of %llu particles exceeds np_alloc %llu (have %llu)",
+           (unsigned long long)n, (unsigned long long)p->np_alloc,
+           (unsigned long long)p->np_local);
+    flush_wrap(p);
+    std::vector<float> tmp(3 * n);
+    for (uint64_t i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) tmp[a * n + i] = (float)x[i * row_stride + a * col_stride];
+    for (int a = 0; a < 3; ++a)
+      FS2_HIP(hipMemcpyAsync(p->x() + a * p->np_alloc + p->np_local, tmp.data() + a * n,
+                             n * sizeof(float), hipMemcpyHostToDevice, stream()));
+    sync();
+    p->np_local += n;
+    p->np_total = p->np_local;
+  });
+}
+
+int fs2_particles_clear(fs2_particles* p) {
+  return guard([&] {
+    check_p(p);
+    p->wrap_pending = false;
+    p->np_local = 0;
+    p->np_total = 0;
+  });
+}
+
+int fs2_particles_get_field(fs2_particles* p, int field, float* out) {
+  return guard([&] {
+    check_p(p);
+    const float* d = nullptr;
+    switch (field) {
+      case FS2_FIELD_X: flush_wrap(p); d = p->x(); break;
+      case FS2_FIELD_DX1: d = p->dx1(); break;
+      case FS2_FIELD_DX2: d = p->dx2(); break;
+      case FS2_FIELD_V: d = p->v(); break;
+      case FS2_FIELD_FORCE: d = p->f(); break;
+      default: fail(FS2_ERR_VALUE, "unknown particle field %d", field);
+    }
+    download_rows(d, p->np_alloc, p->np_local, out);
+  });
+}
+
+int fs2_particles_set_field(fs2_particles* p, int field, const float* in) {
+  return guard([&] {
+    check_p(p);
+    float* d = nullptr;
+    switch (field) {
+      case FS2_FIELD_X: flush_wrap(p); d = p->x(); break;
+      case FS2_FIELD_DX1: d = p->dx1(); break;
+      case FS2_FIELD_DX2: d = p->dx2(); break;
+      case FS2_FIELD_V: d = p->v(); break;
+      case FS2_FIELD_FORCE: d = p->f(); break;
+      default: fail(FS2_ERR_VALUE, "unknown particle field %d", field);
+    }
+    upload_rows(d, p->np_alloc, p->np_local, in);
+  });
+}
+
+int fs2_particles_get_id(fs2_particles* p, uint64_t* out) {
+  return guard([&] {
+    check_p(p);
+    if (p->np_local == 0) return;
+    FS2_HIP(hipMemcpyAsync(out, p->id, p->np_local * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           stream()));
+    sync();
+  });
+}
+
+int fs2_particles_get_records(fs2_particles* p, uint64_t start, uint64_t count, void* out) {
+  return guard([&] {
+    check_p(p);
+    if (start > p->np_local || count > p->np_local - start)
+      fail(FS2_ERR_INDEX, "record range [%llu, %llu) outside [0, %llu)",
+           (unsigned long long)start, (unsigned long long)(start + count),
+           (unsigned long long)p->np_local);
+    if (count == 0) return;
+    flush_wrap(p);
+    // py/particle.h:12-18: x v dx1 dx2 (float[3] each) + u64 id = 56 bytes
+    std::vector<float> f(12 * count);
+    std::vector<uint64_t> ids(count);
+    const float* src[4] = {p->x(), p->v(), p->dx1(), p->dx2()};
+    for (int k = 0; k < 4; ++k)
+      for (int a = 0; a < 3; ++a)
+        FS2_HIP(hipMemcpyAsync(f.data() + (3 * k + a) * count, src[k] + a * p->np_alloc + start,
+                               count * sizeof(float), hipMemcpyDeviceToHost, stream()));
+    FS2_HIP(hipMemcpyAsync(ids.data(), p->id + start, count * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, stream()));
+    sync();
+    unsigned char* o = (unsigned char*)out;
+    for (uint64_t i = 0; i < count; ++i) {
+      float rec[12];
+      for (int c = 0; c < 12; ++c) rec[c] = f[c * count + i];
+      std::memcpy(o + 56 * i, rec, 48);
+      std::memcpy(o + 56 * i + 48, &ids[i], 8);
+    }
+  });
+}
+
+int fs2_particles_periodic_wrapup(fs2_particles* p) {
+  return guard([&] {
+    check_p(p);
+    p->wrap_pending = false;
+    wrap_positions(pos_of(p), (float)p->boxsize);
+  });
+}
+
+int fs2_particles_get_times(const fs2_particles* p, double* a_x, double* a_v, double* a_f) {
+  return guard([&] {
+    *a_x = p->a_x;
+    *a_v = p->a_v;
+    *a_f = p->a_f;
+  });
+}
+
+int fs2_particles_set_times(fs2_particles* p, double a_x, double a_v, double a_f) {
+  return guard([&] {
+    p->a_x = a_x;
+    p->a_v = a_v;
+    p->a_f = a_f;
+  });
+}
+
+int fs2_particles_device_fields(fs2_particles* p, float** x, float** v, float** dx1, float** dx2,
+                                float** force, uint64_t** id, uint64_t* stride) {
+  return guard([&] {
+    check_p(p);
+    flush_wrap(p);
+    if (x) *x = p->x();
+    if (v) *v = p->v();
+    if (dx1) *dx1 = p->dx1();
+    if (dx2) *dx2 = p->dx2();
+    if (force) *force = p->f();
+    if (id) *id = p->id;
+    if (stride) *stride = p->np_alloc;
+  });
+}
+
+int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src) {
+  return guard([&] {
+    if (!dst || !src) fail(FS2_ERR_VALUE, "null particles handle");
+    if (dst->np_alloc != src->np_alloc) fail(FS2_ERR_VALUE, "np_alloc mismatch in copy");
+    FS2_HIP(hipMemcpyAsync(dst->planes, src->planes, (size_t)15 * src->np_alloc * sizeof(float),
+                           hipMemcpyDeviceToDevice, stream()));
+    FS2_HIP(hipMemcpyAsync(dst->id, src->id, src->np_alloc * sizeof(uint64_t),
+                           hipMemcpyDeviceToDevice, stream()));
+    dst->np_local = src->np_local;
+    dst->np_total = src->np_total;
+    dst->boxsize = src->boxsize;
+    dst->a_x = src->a_x;
+    dst->a_v = src->a_v;
+    dst->a_f = src->a_f;
+    dst->wrap_pending = src->wrap_pending;
+  });
+}
+
+// ---- grids -------------------------------------------------------------------------------------------------
+int fs2_grid_alloc(int nc, fs2_grid** out) {
+  return guard([&] { *out = grid_new(nc); });
+}
+void fs2_grid_free(fs2_grid* g) {
+  if (!g || !g->owning) return;
+  if (g_inited) (void)hipStreamSynchronize(g_stream);
+  dfree(g->d);
+  delete g;
+}
+int fs2_grid_nc(const fs2_grid* g, int* nc) {
+  return guard([&] { *nc = g->nc; });
+}
+int fs2_grid_set_test_data(fs2_grid* g) {
+  return guard([&] {
+    // py/py_fft.cpp:57-85: fx[ix,iy,iz] = (ix nc + iy) nc + iz + 1
+    const int n = g->nc;
+    std::vector<float> h((size_t)n * n * n);
+    for (size_t i = 0; i < h.size(); ++i) h[i] = (float)(i + 1);
+    grid_set_real(g, h.data());
+  });
+}
+int fs2_grid_get_real(fs2_grid* g, float* out) {
+  return guard([&] { grid_get_real(g, out); });
+}
+int fs2_grid_set_real(fs2_grid* g, const float* in) {
+  return guard([&] { grid_set_real(g, in); });
+}
+int fs2_grid_get_complex(fs2_grid* g, float* out) {
+  return guard([&] {
+    FS2_HIP(hipMemcpyAsync(out, g->d, grid_floats(g->nc) * sizeof(float), hipMemcpyDeviceToHost,
+                           stream()));
+    sync();
+  });
+}
+int fs2_grid_set_complex(fs2_grid* g, const float* in) {
+  return guard([&] {
+    FS2_HIP(hipMemcpyAsync(g->d, in, grid_floats(g->nc) * sizeof(float), hipMemcpyHostToDevice,
+                           stream()));
+    sync();
+  });
+}
+int fs2_grid_r2c(fs2_grid* g) {
+  return guard([&] { fft_r2c_3d(g->d, g->nc); });
+}
+int fs2_grid_c2r(fs2_grid* g) {
+  return guard([&] { fft_c2r_3d(g->d, g->nc); });
+}
+
+// ---- PM ---------------------------------------------------------------------------------------------------------
+int fs2_pm_create(int nc_pm, double pm_factor, double boxsize, fs2_pm** out) {
+  return guard([&] {
+    // py/pm.cpp:245-255
+    if (nc_pm <= 1) fail(FS2_ERR_RUNTIME, "nc_pm (= %d) must be larger than 1", nc_pm);
+    if (boxsize <= 0) fail(FS2_ERR_RUNTIME, "boxsize (= %f) must be positive", boxsize);
+    if (!fft_supported(nc_pm)) fail(FS2_ERR_VALUE, "nc_pm = %d is not a supported FFT size", nc_pm);
+    fs2_pm* pm = new fs2_pm();
+    pm->nc = nc_pm;
+    pm->pm_factor = pm_factor;
+    pm->boxsize = boxsize;
+    pm->boxsize_f = (float)boxsize;
+    const size_t bytes = grid_floats(nc_pm) * sizeof(float);
+    try {
+      pm->mesh = (float*)dmalloc(bytes);
+      for (int a = 0; a < 3; ++a) pm->force[a] = (float*)dmalloc(bytes);
+    } catch (...) {
+      dfree(pm->mesh);
+      for (int a = 0; a < 3; ++a) dfree(pm->force[a]);
+      delete pm;
+      throw;
+    }
+    pm->density.nc = nc_pm;
+    pm->density.d = pm->mesh;
+    pm->density.owning = false;
+    *out = pm;
+  });
+}
+
+void fs2_pm_destroy(fs2_pm* pm) {
+  if (!pm) return;
+  if (g_inited) (void)hipStreamSynchronize(g_stream);
+  dfree(pm->mesh);
+  for (int a = 0; a < 3; ++a) dfree(pm->force[a]);
+  delete pm;
+}
+
+int fs2_pm_params(const fs2_pm* pm, int* nc, double* pm_factor, double* boxsize) {
+  return guard([&] {
+    *nc = pm->nc;
+    *pm_factor = pm->pm_factor;
+    *boxsize = pm->boxsize;
+  });
+}
+
+int fs2_pm_set_paint_variant(fs2_pm* pm, int variant) {
+  return guard([&] {
+    if (variant != FS2_PAINT_ATOMIC)
+      fail(FS2_ERR_VALUE, "paint variant %d not available", variant);
+    pm->variant = variant;
+  });
+}
+
+int fs2_pm_send_positions(fs2_pm* pm, fs2_particles* p) {
+  return guard([&] {
+    (void)pm;
+    check_p(p);
+    p->wrap_pending = true;
+  });
+}
+
+int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density) {
+  return guard([&] {
+    check_p(p);
+    const int n = pm->nc;
+    // pm.cpp:414-426 clear to -1, then pm.cpp:57-139 (fac = pm_factor^3)
+    fill_grid(pm->mesh, n, -1.0f);
+    const float fac = (float)(pm->pm_factor * pm->pm_factor * pm->pm_factor);
+    paint_atomic(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending);
+    p->wrap_pending = false;
+    pm->mesh_is_k = false;
+    pm->status = PM_DENSITY_DONE;
+    if (density) *density = &pm->density;
+  });
+}
+
+int fs2_pm_check_total_density(fs2_pm* pm, double* sum) {
+  return guard([&] {
+    if (pm->mesh_is_k || pm->status != PM_DENSITY_DONE)
+      fail(FS2_ERR_RUNTIME, "density mesh not available for the total-density check");
+    const double s = grid_sum(pm->mesh, pm->nc);
+    if (sum) *sum = s;
+    const double n = pm->nc;
+    const double tol = 10.0 * 1.1920928955078125e-07 * n * n * n;   // pm.cpp:376
+    if (std::fabs(s) > tol)
+      fail(FS2_ERR_ASSERTION, "total CIC density error is too large: %le > %le", s, tol);
+  });
+}
+
+int fs2_pm_compute_force(fs2_pm* pm, fs2_particles* p) {
+  return guard([&] {
+    check_p(p);
+    if (p->a_f == p->a_x) return;   // pm.cpp:311-312
+    if (pm->status != PM_DENSITY_DONE) fail(FS2_ERR_RUNTIME, "PM density not ready");
+    const int n = pm->nc;
+    // pm.cpp:472: f1 = -1/n^3 / (2 pi / boxsize), boxsize held as a Float
+    const float f1 = (float)(-1.0 / std::pow((double)n, 3.0) / (2.0 * M_PI / (double)pm->boxsize_f));
+    fft_r2c_3d(pm->mesh, n);
+    pm->mesh_is_k = true;
+    for (int a = 0; a < 3; ++a) fft_green_c2r_3d(pm->mesh, pm->force[a], n, a, f1);
+    flush_wrap(p);
+    readout3(pos_of(p), pm->force[0], pm->force[1], pm->force[2], p->f(), n, pm->boxsize_f);
+    p->a_f = p->a_x;
+    pm->status = PM_FORCE_DONE;
+  });
+}
+
+int fs2_pm_get_forces(fs2_pm* pm, fs2_particles* p) {
+  return guard([&] {
+    (void)pm;
+    check_p(p);
+  });
+}
+
+int fs2_pm_get_force_mesh(fs2_pm* pm, int axis, float* out) {
+  return guard([&] {
+    if (axis < 0 || axis > 2) fail(FS2_ERR_VALUE, "axis %d", axis);
+    fs2_grid view;
+    view.nc = pm->nc;
+    view.d = pm->force[axis];
+    view.owning = false;
+    grid_get_real(&view, out);
+  });
+}
+
+int fs2_pm_power_spectrum(fs2_pm* pm, int nbins, double dk_bin, double* k, double* P,
+                          double* nmodes) {
+  return guard([&] {
+    if (pm->status != PM_DENSITY_DONE && !pm->mesh_is_k)
+      fail(FS2_ERR_RUNTIME, "no density mesh for the power spectrum");
+    if (!pm->mesh_is_k) {
+      fft_r2c_3d(pm->mesh, pm->nc);
+      pm->mesh_is_k = true;
+    }
+    power_spectrum(pm->mesh, pm->nc, pm->boxsize, nbins, dk_bin, k, P, nmodes);
+  });
+}
+
+// ---- COLA ---------------------------------------------------------------------------------------------------------
+int fs2_cola_kick(fs2_particles* p, double a_vel) {
+  return guard([&] {
+    check_p(p);
+    double kf, q1, q2;
+    int rc = fs2_cola_kick_coefficients(p->a_v, p->a_x, a_vel, &kf, &q1, &q2);
+    if (rc) throw Error{rc, g_err};
+    const double om = host::omega_m();
+    // cola.cpp:57, 65-66: kick_factor, q1, q2 stored as Float
+    cola_kick(p->v(), p->f(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local,
+              -1.5 * om, (float)kf, (float)q1, (float)q2);
+    p->a_v = a_vel;
+  });
+}
+
+int fs2_cola_drift(fs2_particles* p, double a_pos) {
+  return guard([&] {
+    check_p(p);
+    double dt, da1, da2;
+    int rc = fs2_cola_drift_coefficients(p->a_x, p->a_v, a_pos, &dt, &da1, &da2);
+    if (rc) throw Error{rc, g_err};
+    flush_wrap(p);
+    // cola.cpp:106-109: da1, da2 stored as Float; dt stays double
+    cola_drift(p->x(), p->v(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local, dt,
+               (float)da1, (float)da2);
+    p->a_x = a_pos;
+  });
+}
+
+int fs2_cola_velocity(fs2_particles* p, float* out) {
+  return guard([&] {
+    check_p(p);
+    const double a = p->a_x;
+    const double D1 = host::D_growth(a);
+    const float Dv = (float)host::Dv_growth(a, D1);
+    const float D2v = (float)host::D2v_growth(a, D1);   // cola.cpp:138 passes D1
+    // the kernel writes planes at stride np_alloc
+    float* full = (float*)dmalloc(3 * std::max<uint64_t>(p->np_alloc, 1) * sizeof(float));
+    try {
+      cola_velocity(full, p->v(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local,
+                    Dv, D2v);
+      download_rows(full, p->np_alloc, p->np_local, out);
+    } catch (...) {
+      dfree(full);
+      throw;
+    }
+    dfree(full);
+  });
+}
+
+// ---- 2LPT ---------------------------------------------------------------------------------------------------------
+int fs2_lpt_set_offset(double offset) {
+  return guard([&] { lpt_offset = (float)offset; });
+}
+
+int fs2_lpt_set_zeldovich_force(fs2_particles* p, double a) {
+  return guard([&] {
+    check_p(p);
+    const float g1 = (float)host::D_growth(a);   // py_lpt.cpp:74
+    scale_field(p->f(), p->dx1(), (int64_t)p->np_alloc, (int64_t)p->np_local, g1);
+  });
+}
+
+int fs2_lpt_white_noise(int nc, uint64_t seed, float* out) {
+  return guard([&] {
+    fs2_grid* g = grid_new(nc);
+    try {
+      white_noise(g->d, nc, seed);
+      grid_get_real(g, out);
+    } catch (...) {
+      fs2_grid_free(g);
+      throw;
+    }
+    fs2_grid_free(g);
+  });
+}
+
+int fs2_lpt(int nc, double boxsize, double a, uint64_t seed, const fs2_power* ps, const char* kind,
+            fs2_particles** out) {
+  return guard([&] {
+    if (!ps) fail(FS2_ERR_VALUE, "null power spectrum");
+    const std::string k(kind ? kind : "");
+    if (k != "cola" && k != "zeldovich" && k != "2lpt")
+      fail(FS2_ERR_VALUE, "unknown LPT kind '%s' (zeldovich, 2lpt, cola)", k.c_str());
+    if (nc < 4 || nc % 4 != 0 || !fft_supported(nc))
+      fail(FS2_ERR_VALUE, "nc = %d not supported for 2LPT", nc);
+    if (boxsize <= 0) fail(FS2_ERR_VALUE, "boxsize must be positive");
+    (void)host::omega_m();   // cosmology must be initialised
+
+    // amplitude table indexed by integer |k|^2 (non-Nyquist modes only)
+    const int h = nc / 2;
+    const int nk2 = 3 * h * h + 1;
+    std::vector<double> amp(nk2, 0.0);
+    const double kf = 2.0 * M_PI / boxsize;
+    const double V = boxsize * boxsize * boxsize;
+    const double norm = 1.0 / std::pow((double)nc, 1.5);
+    std::vector<char> used(nk2, 0);
+    for (int ix = -h + 1; ix < h; ++ix)
+      for (int iy = -h + 1; iy < h; ++iy)
+        for (int iz = 0; iz < h; ++iz) used[ix * ix + iy * iy + iz * iz] = 1;
+    for (int q = 1; q < nk2; ++q)
+      if (used[q]) amp[q] = std::sqrt(power_P(ps, kf * std::sqrt((double)q)) / V) * norm;
+
+    const size_t gbytes = grid_floats(nc) * sizeof(float);
+    std::vector<float*> bufs;
+    double* d_amp = nullptr;
+    fs2_particles* p = nullptr;
+    auto cleanup = [&] {
+      for (float* b : bufs) dfree(b);
+      dfree(d_amp);
+    };
+    try {
+      for (int i = 0; i < 7; ++i) bufs.push_back((float*)dmalloc(gbytes));
+      d_amp = (double*)dmalloc(nk2 * sizeof(double));
+      FS2_HIP(hipMemcpyAsync(d_amp, amp.data(), nk2 * sizeof(double), hipMemcpyHostToDevice,
+                             stream()));
+      float* S = bufs[0];
+      float* G[6] = {bufs[1], bufs[2], bufs[3], bufs[4], bufs[5], bufs[6]};
+      white_noise(S, nc, seed);
+      fft_r2c_3d(S, nc);
+      lpt_psi_ij(S, G, nc, boxsize, d_amp);
+      for (int i = 0; i < 6; ++i) fft_c2r_3d(G[i], nc);
+      const float* cg[6] = {G[0], G[1], G[2], G[3], G[4], G[5]};
+      lpt_div2(G[0], cg, nc);
+      fft_r2c_3d(G[0], nc);
+      float* o5[5] = {G[1], G[2], G[3], G[4], G[5]};
+      lpt_psi12(S, G[0], o5, nc, boxsize, d_amp);
+      fft_c2r_3d(G[1], nc);
+      fft_c2r_3d(G[2], nc);
+      fft_c2r_3d(S, nc);
+      fft_c2r_3d(G[3], nc);
+      fft_c2r_3d(G[4], nc);
+      fft_c2r_3d(G[5], nc);
+
+      // lpt.cpp:156-175: Float D1; D2, Dv, D2v evaluated from the Float D1
+      const float D1 = (float)host::D_growth(a);
+      const float D2 = k == "zeldovich" ? 0.f : (float)host::D2_growth(a, (double)D1);
+      float Dv = 0.f, D2v = 0.f;
+      if (k == "zeldovich") {
+        Dv = (float)host::Dv_growth(a, (double)D1);
+      } else if (k == "2lpt") {
+        Dv = (float)host::Dv_growth(a, (double)D1);
+        D2v = (float)host::D2v_growth(a, (double)D1);   // lpt.cpp:172 passes D1
+      }
+      const uint64_t np = (uint64_t)nc * nc * nc;
+      int rc = fs2_particles_alloc(np, boxsize, &p);
+      if (rc) throw Error{rc, g_err};
+      LatticeArgs la{p->x(), p->v(), p->dx1(), p->dx2(), p->id, (int64_t)p->np_alloc};
+      const float* psi1[3] = {G[1], G[2], S};
+      const float* psi2[3] = {G[3], G[4], G[5]};
+      const float dx = (float)(boxsize / nc);   // lpt.cpp:144
+      lpt_lattice(la, psi1, psi2, nc, dx, lpt_offset, D1, D2, Dv, D2v,
+                  1.0 / std::pow((double)nc, 3.0));
+      sync();
+      p->np_local = np;
+      p->np_total = np;
+      p->a_x = a;
+      p->a_v = a;
+    } catch (...) {
+      cleanup();
+      fs2_particles_free(p);
+      throw;
+    }
+    cleanup();
+    *out = p;
+  });
+}
+
+// ---- kernel timing ---------------------------------------------------------------------------------------------
+int fs2_profile_enable(int on) {
+  return guard([&] {
+    if (!on) prof_resolve();
+    g_prof = on != 0;
+  });
+}
+int fs2_profile_reset(void) {
+  return guard([&] {
+    prof_resolve();
+    g_recs.clear();
+  });
+}
+int fs2_profile_count(int* n) {
+  return guard([&] {
+    prof_resolve();
+    *n = (int)g_recs.size();
+  });
+}
+int fs2_profile_get(int i, const char** name, uint64_t* launches, double* total_ms,
+                    double* bytes_per_launch) {
+  return guard([&] {
+    prof_resolve();
+    if (i < 0 || i >= (int)g_recs.size()) fail(FS2_ERR_INDEX, "profile index %d", i);
+    *name = g_recs[i].name.c_str();
+    *launches = g_recs[i].launches;
+    *total_ms = g_recs[i].total_ms;
+    *bytes_per_launch = g_recs[i].bytes;
+  });
+}
+
+}  // extern "C"

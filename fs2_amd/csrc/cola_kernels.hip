@@ -1,0 +1,98 @@
+// COLA kick / drift kernels for gfx950 (py/cola.cpp:48-148).
+//
+// The FP64 time coefficients come from the host; the per-particle update
+// keeps the reference's mixed-precision expression order, so contraction is
+// disabled here.
+#include "fs2_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace fs2 {
+namespace {
+
+constexpr int kBlock = 256;
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// cola.cpp:79-88:  Float a = -1.5*om*(f + dx1*q1 + dx2*q2)  (double product,
+// rounded to float), then v += a*kick_factor in float.
+__global__ void k_kick(float* __restrict__ v, const float* __restrict__ f,
+                       const float* __restrict__ dx1, const float* __restrict__ dx2,
+                       int64_t stride, int64_t np, double m15om, float kf, float q1, float q2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t j = i + a * stride;
+    const float s = f[j] + dx1[j] * q1 + dx2[j] * q2;
+    const float acc = (float)(m15om * (double)s);
+    v[j] = v[j] + acc * kf;
+  }
+}
+
+// cola.cpp:117-124:  x += v*dt + (dx1*da1 + dx2*da2)  with dt double, so the
+// sum is formed in double and rounded once into x.
+__global__ void k_drift(float* __restrict__ x, const float* __restrict__ v,
+                        const float* __restrict__ dx1, const float* __restrict__ dx2,
+                        int64_t stride, int64_t np, double dt, float da1, float da2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t j = i + a * stride;
+    const float lpt = dx1[j] * da1 + dx2[j] * da2;
+    const double rhs = (double)v[j] * dt + (double)lpt;
+    x[j] = (float)((double)x[j] + rhs);
+  }
+}
+
+// cola.cpp:141-145: v + Dv dx1 + D2v dx2 (Float coefficients).
+__global__ void k_velocity(float* __restrict__ out, const float* __restrict__ v,
+                           const float* __restrict__ dx1, const float* __restrict__ dx2,
+                           int64_t stride, int64_t np, float Dv, float D2v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t j = i + a * stride;
+    out[j] = v[j] + Dv * dx1[j] + D2v * dx2[j];
+  }
+}
+
+__global__ void k_scale(float* __restrict__ out, const float* __restrict__ in, int64_t stride,
+                        int64_t np, float s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[i + a * stride] = s * in[i + a * stride];
+}
+
+}  // namespace
+
+void cola_kick(float* v, const float* f, const float* dx1, const float* dx2, int64_t stride,
+               int64_t np, double m15om, float kf, float q1, float q2) {
+  if (np == 0) return;
+  launch("cola_kick", (double)np * 60.0, k_kick, dim3(blocks_for(np)), dim3(kBlock), 0, v, f,
+         dx1, dx2, stride, np, m15om, kf, q1, q2);
+}
+
+void cola_drift(float* x, const float* v, const float* dx1, const float* dx2, int64_t stride,
+                int64_t np, double dt, float da1, float da2) {
+  if (np == 0) return;
+  launch("cola_drift", (double)np * 60.0, k_drift, dim3(blocks_for(np)), dim3(kBlock), 0, x, v,
+         dx1, dx2, stride, np, dt, da1, da2);
+}
+
+void cola_velocity(float* out, const float* v, const float* dx1, const float* dx2,
+                   int64_t stride, int64_t np, float Dv, float D2v) {
+  if (np == 0) return;
+  launch("cola_velocity", (double)np * 48.0, k_velocity, dim3(blocks_for(np)), dim3(kBlock), 0,
+         out, v, dx1, dx2, stride, np, Dv, D2v);
+}
+
+void scale_field(float* out, const float* in, int64_t stride, int64_t np, float s) {
+  if (np == 0) return;
+  launch("scale_field", (double)np * 24.0, k_scale, dim3(blocks_for(np)), dim3(kBlock), 0, out,
+         in, stride, np, s);
+}
+
+}  // namespace fs2

@@ -1,0 +1,472 @@
+// Hand-written 3D r2c / c2r FFTs for gfx950.
+//
+// Replaces FFTW3f(-MPI) as used by py/fft.cpp:19-100 (unnormalised, sign -1
+// forward, in place on a padded [n][n][2(n/2+1)] real grid).  The transform
+// is three HBM passes, each a batch of 1D FFTs staged through LDS:
+//
+//   forward r2c:  z pass (rows, r2c via an n/2-point complex FFT)
+//                 y pencils (stride n/2+1), x pencils (stride n(n/2+1))
+//   inverse c2r:  x pencils, y pencils, z pass (c2r)
+//
+// The inverse runs the complex passes first and the c2r along z last, so the
+// imaginary parts of the kz = 0 and kz = n/2 planes are dropped after the x/y
+// transforms -- the FFTW / numpy irfftn result for the non-Hermitian planes
+// that multiplying by an odd function of k produces.
+//
+// Inside a pass every pencil is transformed by a Stockham autosort FFT in LDS
+// (Govindaraju et al., SC'08 formulation): stage s with radix R and span Ns
+// reads v[r] = in[j + r N/R], multiplies by w^{r (j mod Ns)}, w =
+// e^{-2 pi i/(Ns R)}, applies a radix-R DFT, and writes
+// out[(j/Ns) Ns R + j mod Ns + r Ns].  Radices 2, 3, 4, 8, 16; twiddles come
+// from an FP64-computed table rounded once to FP32.  Inverse transforms use
+// IDFT(x) = conj(DFT(conj(x))).
+#include <cmath>
+#include <map>
+#include <mutex>
+
+#include "fs2_internal.h"
+
+namespace fs2 {
+namespace {
+
+// ---- complex helpers -------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 mul_negi(float2 a) { return make_float2(a.y, -a.x); }   // -i a
+__device__ __forceinline__ float2 mul_posi(float2 a) { return make_float2(-a.y, a.x); }   //  i a
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// ---- radix-R DFTs, forward sign ----------------------------------------------------
+template <int R> struct DFT;
+
+template <> struct DFT<2> {
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <> struct DFT<3> {
+  static __device__ __forceinline__ void run(float2* v) {
+    const float c = -0.5f, s = 0.86602540378443864676f;   // w = c - i s
+    float2 a = v[0], sum = cadd(v[1], v[2]), dif = csub(v[1], v[2]);
+    float2 m = make_float2(a.x + c * sum.x, a.y + c * sum.y);
+    float2 r = make_float2(s * dif.y, -s * dif.x);         // -i s (x1 - x2)
+    v[0] = cadd(a, sum);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+
+template <> struct DFT<4> {
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = mul_negi(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <> struct DFT<8> {
+  static __device__ __forceinline__ void run(float2* v) {
+    const float h = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    DFT<4>::run(e);
+    DFT<4>::run(o);
+    // w8^k, k = 0..3 : 1, (1-i)/sqrt2, -i, (-1-i)/sqrt2
+    float2 o1 = make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    float2 o2 = mul_negi(o[2]);
+    float2 o3 = make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+  }
+};
+
+template <> struct DFT<16> {
+  static __device__ __forceinline__ void run(float2* v) {
+    // 4 x 4: A_r = DFT4(x_r, x_{r+4}, x_{r+8}, x_{r+12});
+    // y[k1 + 4 k2] = DFT4_r( w16^{r k1} A_r[k1] )[k2]
+    float2 a[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      a[r][0] = v[r]; a[r][1] = v[r + 4]; a[r][2] = v[r + 8]; a[r][3] = v[r + 12];
+      DFT<4>::run(a[r]);
+    }
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    const float h = 0.70710678118654752440f;
+    // w16^m for m = 0..9 (r k1 <= 9): cos(2 pi m/16) - i sin(2 pi m/16)
+    const float2 w[10] = {
+        make_float2(1.f, 0.f),  make_float2(c1, -s1), make_float2(h, -h),   make_float2(s1, -c1),
+        make_float2(0.f, -1.f), make_float2(-s1, -c1), make_float2(-h, -h), make_float2(-c1, -s1),
+        make_float2(-1.f, 0.f), make_float2(-c1, s1)};
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      float2 b[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) b[r] = (r * k1 == 0) ? a[r][k1] : cmul(a[r][k1], w[r * k1]);
+      DFT<4>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+    }
+  }
+};
+
+// ---- plans -----------------------------------------------------------------------------
+template <int... Rs> struct Radices {};
+template <int N> struct Plan;
+// T = threads per pencil; every stage radix R must satisfy N % (R T) == 0.
+template <> struct Plan<2>    { static constexpr int T = 1;  using R = Radices<2>; };
+template <> struct Plan<4>    { static constexpr int T = 1;  using R = Radices<4>; };
+template <> struct Plan<8>    { static constexpr int T = 1;  using R = Radices<8>; };
+template <> struct Plan<16>   { static constexpr int T = 4;  using R = Radices<4, 4>; };
+template <> struct Plan<24>   { static constexpr int T = 2;  using R = Radices<4, 2, 3>; };
+template <> struct Plan<32>   { static constexpr int T = 4;  using R = Radices<8, 4>; };
+template <> struct Plan<48>   { static constexpr int T = 4;  using R = Radices<4, 4, 3>; };
+template <> struct Plan<64>   { static constexpr int T = 8;  using R = Radices<8, 8>; };
+template <> struct Plan<96>   { static constexpr int T = 4;  using R = Radices<8, 4, 3>; };
+template <> struct Plan<128>  { static constexpr int T = 8;  using R = Radices<16, 8>; };
+template <> struct Plan<256>  { static constexpr int T = 16; using R = Radices<16, 16>; };
+template <> struct Plan<512>  { static constexpr int T = 64; using R = Radices<8, 8, 8>; };
+template <> struct Plan<1024> { static constexpr int T = 64; using R = Radices<16, 16, 4>; };
+
+// One Stockham stage over one pencil held in LDS at s[j * SJ].
+template <int N, int T, int SJ, int Ns, int R>
+__device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* __restrict__ tw) {
+  static_assert(N % (R * T) == 0, "plan: stage does not divide evenly");
+  constexpr int B = N / (R * T);
+  float2 v[B][R];
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + b * T;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b][r] = s[(j + r * (N / R)) * SJ];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + b * T;
+    const int k = j % Ns;
+    if constexpr (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * (N / (Ns * R))]);
+    }
+    DFT<R>::run(v[b]);
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) s[(base + r * Ns) * SJ] = v[b][r];
+  }
+  __syncthreads();
+}
+
+template <int N, int T, int SJ, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stockham_stages(float2* s, int t, const float2* __restrict__ tw) {
+  stockham_stage<N, T, SJ, Ns, R>(s, t, tw);
+  if constexpr (sizeof...(Rest) > 0) stockham_stages<N, T, SJ, Ns * R, Rest...>(s, t, tw);
+}
+
+template <int N, int T, int SJ, int... Rs>
+__device__ __forceinline__ void run_plan(float2* s, int t, const float2* __restrict__ tw, Radices<Rs...>) {
+  stockham_stages<N, T, SJ, 1, Rs...>(s, t, tw);
+}
+
+// Forward FFT of length N of the pencil s[j*SJ], j < N, by the T threads t.
+template <int N, int SJ>
+__device__ __forceinline__ void fft_lds(float2* s, int t, const float2* __restrict__ tw) {
+  run_plan<N, Plan<N>::T, SJ>(s, t, tw, typename Plan<N>::R{});
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that
+// consecutive logical tiles run on the same XCD and share its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  const int per = nblocks / 8, rem = nblocks % 8;
+  const int xcd = bid % 8, local = bid / 8;
+  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// ---- Green x gradient multiplier (py/pm.cpp:460-504) ---------------------------------------
+struct GreenArg {
+  int axis;
+  int n;
+  float f1;
+};
+
+__device__ __forceinline__ float2 green_mul(float2 v, int ix, int iy, int iz, const GreenArg& g) {
+  const int n = g.n;
+  const int kx = ix <= n / 2 ? ix : ix - n;
+  const int ky = iy <= n / 2 ? iy : iy - n;
+  if (kx == 0 && ky == 0 && iz == 0) return make_float2(0.f, 0.f);
+  const float k0 = (float)kx, k1 = (float)ky, k2 = (float)iz;
+  const float ka = g.axis == 0 ? k0 : (g.axis == 1 ? k1 : k2);
+  const float f2 = g.f1 / (k0 * k0 + k1 * k1 + k2 * k2) * ka;
+  return make_float2(-f2 * v.y, f2 * v.x);
+}
+
+// ---- kernels --------------------------------------------------------------------------------
+// Strided complex pencils: pencil (o, c) has elements src[o*outer + c + j*elem].
+// A block takes P adjacent columns c of one outer index o and stages the N x P
+// tile in LDS as [j][P], so global accesses are P-wide contiguous segments.
+template <int N, int P, bool INV, bool GREEN>
+__global__ void __launch_bounds__(Plan<N>::T * P)
+k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
+             int64_t outer_stride, int nchunks, int nblocks, const float2* __restrict__ tw,
+             GreenArg g) {
+  constexpr int T = Plan<N>::T;
+  __shared__ float2 s[N * P];
+  const int tid = threadIdx.x;
+  const int p = tid % P, t = tid / P;
+  const int bid = xcd_remap(blockIdx.x, nblocks);
+  const int o = bid / nchunks;
+  const int c = (bid - o * nchunks) * P + p;
+  const bool valid = c < ncols;
+  const int64_t base = (int64_t)o * outer_stride + c;
+#pragma unroll
+  for (int i = 0; i < N / T; ++i) {
+    const int j = t + i * T;
+    float2 v = make_float2(0.f, 0.f);
+    if (valid) v = src[base + (int64_t)j * elem_stride];
+    if constexpr (GREEN) v = green_mul(v, j, o, c, g);
+    if constexpr (INV) v.y = -v.y;
+    s[j * P + p] = v;
+  }
+  __syncthreads();
+  fft_lds<N, P>(s + p, t, tw);
+#pragma unroll
+  for (int i = 0; i < N / T; ++i) {
+    const int j = t + i * T;
+    float2 v = s[j * P + p];
+    if constexpr (INV) v.y = -v.y;
+    if (valid) dst[base + (int64_t)j * elem_stride] = v;
+  }
+}
+
+// z pass, r2c: each row of n = 2M reals (padded to M+1 complex) is read as M
+// complex z[m] = x[2m] + i x[2m+1], transformed, and split:
+//   X[k] = E - i W^k O,  E = (Z[k] + conj Z[M-k])/2,  O = (Z[k] - conj Z[M-k])/2,
+//   W = e^{-2 pi i/n}, k = 0..M.
+template <int M, int P>
+__global__ void __launch_bounds__(Plan<M>::T * P)
+k_fft_z_r2c(float2* data, int64_t nrows, const float2* __restrict__ twM,
+            const float2* __restrict__ twN) {
+  constexpr int T = Plan<M>::T;
+  constexpr int LS = M + 1;
+  __shared__ float2 s[P * LS];
+  const int tid = threadIdx.x;
+  const int t = tid % T, p = tid / T;
+  const int64_t row = (int64_t)blockIdx.x * P + p;
+  const bool valid = row < nrows;
+  float2* r = data + row * (M + 1);
+  float2* sp = s + p * LS;
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int m = t + i * T;
+    sp[m] = valid ? r[m] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<M, 1>(sp, t, twM);
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int k = t + i * T;
+    const float2 A = sp[k];
+    const float2 Bc = cconj(sp[k == 0 ? 0 : M - k]);
+    const float2 E = cscale(cadd(A, Bc), 0.5f);
+    const float2 O = cscale(csub(A, Bc), 0.5f);
+    const float2 X = cadd(E, mul_negi(cmul(twN[k], O)));
+    if (valid) r[k] = X;
+  }
+  if (t == 0 && valid) {
+    const float2 A = sp[0];
+    r[M] = make_float2(A.x - A.y, 0.f);
+  }
+}
+
+// z pass, c2r: the inverse of the split above, with the imaginary parts of
+// X[0] and X[M] dropped:  Z[k] = E' + i O',  E' = X[k] + conj X[M-k],
+// O' = (X[k] - conj X[M-k]) W^{-k};  z = IDFT_M(Z) = x_even + i x_odd.
+template <int M, int P>
+__global__ void __launch_bounds__(Plan<M>::T * P)
+k_fft_z_c2r(float2* data, int64_t nrows, const float2* __restrict__ twM,
+            const float2* __restrict__ twN) {
+  constexpr int T = Plan<M>::T;
+  constexpr int LS = M + 1;
+  __shared__ float2 s[P * LS];
+  const int tid = threadIdx.x;
+  const int t = tid % T, p = tid / T;
+  const int64_t row = (int64_t)blockIdx.x * P + p;
+  const bool valid = row < nrows;
+  float2* r = data + row * (M + 1);
+  float2* sp = s + p * LS;
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int m = t + i * T;
+    sp[m] = valid ? r[m] : make_float2(0.f, 0.f);
+  }
+  if (t == 0) sp[M] = valid ? r[M] : make_float2(0.f, 0.f);
+  __syncthreads();
+  float2 z[M / T];
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int k = t + i * T;
+    float2 A = sp[k];
+    float2 B = sp[M - k];
+    if (k == 0) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    const float2 Bc = cconj(B);
+    const float2 E = cadd(A, Bc);
+    const float2 O = cmul(csub(A, Bc), cconj(twN[k]));
+    z[i] = cconj(cadd(E, mul_posi(O)));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) sp[t + i * T] = z[i];
+  __syncthreads();
+  fft_lds<M, 1>(sp, t, twM);
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int m = t + i * T;
+    if (valid) r[m] = cconj(sp[m]);
+  }
+}
+
+// ---- host side ---------------------------------------------------------------------------------
+const float2* twiddles(int N) {
+  static std::mutex mu;
+  static std::map<int, float2*> tables;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = tables.find(N);
+  if (it != tables.end()) return it->second;
+  std::vector<float2> h(N);
+  for (int k = 0; k < N; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)N;
+    h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  float2* d = (float2*)dmalloc(sizeof(float2) * N);
+  FS2_HIP(hipMemcpy(d, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+  tables[N] = d;
+  return d;
+}
+
+// Columns per block for the strided passes, and rows per block for the z pass.
+template <int N> constexpr int pencil_cols() {
+  return (N * 16 * 8 <= 65536 && Plan<N>::T * 16 <= 1024) ? 16 : 8;
+}
+template <int M> constexpr int zrows() {
+  return Plan<M>::T >= 256 ? 1 : 256 / Plan<M>::T;
+}
+
+template <int n>
+struct Impl {
+  static constexpr int N = n;
+  static constexpr int M = n / 2;
+  static constexpr int P = pencil_cols<N>();
+  static constexpr int PZ = zrows<M>();
+
+  static double grid_bytes() { return (double)n * n * (n / 2 + 1) * 8.0; }
+
+  template <bool INV, bool GREEN>
+  static void pencil(const char* name, const float2* src, float2* dst, int64_t elem_stride,
+                     int64_t outer_stride, GreenArg g) {
+    const int nh = n / 2 + 1;
+    const int nchunks = (nh + P - 1) / P;
+    const int nblocks = n * nchunks;
+    launch(name, 2.0 * grid_bytes(), k_fft_pencil<N, P, INV, GREEN>, dim3(nblocks),
+           dim3(Plan<N>::T * P), 0, src, dst, nh, elem_stride, outer_stride, nchunks, nblocks,
+           twiddles(N), g);
+  }
+
+  static void z_r2c(float* grid) {
+    const int64_t nrows = (int64_t)n * n;
+    const double bytes = (double)n * n * n * 4.0 + grid_bytes();
+    launch("fft_z_r2c", bytes, k_fft_z_r2c<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
+           dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, twiddles(M), twiddles(N));
+  }
+
+  static void z_c2r(float* grid) {
+    const int64_t nrows = (int64_t)n * n;
+    const double bytes = (double)n * n * n * 4.0 + grid_bytes();
+    launch("fft_z_c2r", bytes, k_fft_z_c2r<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
+           dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, twiddles(M), twiddles(N));
+  }
+
+  static void r2c(float* grid) {
+    const int64_t nh = n / 2 + 1;
+    float2* c = (float2*)grid;
+    GreenArg g{0, n, 0.f};
+    z_r2c(grid);
+    pencil<false, false>("fft_y_fwd", c, c, nh, (int64_t)n * nh, g);
+    pencil<false, false>("fft_x_fwd", c, c, (int64_t)n * nh, nh, g);
+  }
+
+  static void c2r(float* grid) {
+    const int64_t nh = n / 2 + 1;
+    float2* c = (float2*)grid;
+    GreenArg g{0, n, 0.f};
+    pencil<true, false>("fft_x_inv", c, c, (int64_t)n * nh, nh, g);
+    pencil<true, false>("fft_y_inv", c, c, nh, (int64_t)n * nh, g);
+    z_c2r(grid);
+  }
+
+  static void green_c2r(const float* dk, float* dst, int axis, float f1) {
+    const int64_t nh = n / 2 + 1;
+    float2* c = (float2*)dst;
+    GreenArg g{axis, n, f1};
+    pencil<true, true>("fft_x_inv_green", (const float2*)dk, c, (int64_t)n * nh, nh, g);
+    pencil<true, false>("fft_y_inv", c, c, nh, (int64_t)n * nh, g);
+    z_c2r(dst);
+  }
+};
+
+template <typename F>
+void dispatch(int n, F&& f) {
+  switch (n) {
+    case 4: f(Impl<4>{}); break;
+    case 8: f(Impl<8>{}); break;
+    case 16: f(Impl<16>{}); break;
+    case 32: f(Impl<32>{}); break;
+    case 48: f(Impl<48>{}); break;
+    case 64: f(Impl<64>{}); break;
+    case 96: f(Impl<96>{}); break;
+    case 128: f(Impl<128>{}); break;
+    case 256: f(Impl<256>{}); break;
+    case 512: f(Impl<512>{}); break;
+    case 1024: f(Impl<1024>{}); break;
+    default: fail(FS2_ERR_VALUE, "FFT size %d not supported (4..1024: 2^k, 48, 96)", n);
+  }
+}
+
+}  // namespace
+
+bool fft_supported(int n) {
+  switch (n) {
+    case 4: case 8: case 16: case 32: case 48: case 64: case 96: case 128: case 256:
+    case 512: case 1024:
+      return true;
+    default:
+      return false;
+  }
+}
+
+void fft_r2c_3d(float* grid, int n) {
+  dispatch(n, [&](auto impl) { decltype(impl)::r2c(grid); });
+}
+
+void fft_c2r_3d(float* grid, int n) {
+  dispatch(n, [&](auto impl) { decltype(impl)::c2r(grid); });
+}
+
+void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f1) {
+  dispatch(n, [&](auto impl) { decltype(impl)::green_c2r(delta_k, dst, axis, f1); });
+}
+
+}  // namespace fs2

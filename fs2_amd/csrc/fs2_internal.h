@@ -1,0 +1,129 @@
+// Internal declarations shared by the HIP translation units of libfs2hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fs2hip.h"
+
+// Linear power spectrum: a "k P" table (natural cubic spline of ln P in
+// ln k, py/power.cpp:15-54) or the Eisenstein-Hu no-wiggle form.
+struct fs2_power {
+  int kind = 0;   // 0 table, 1 EH98 no-wiggle
+  std::vector<double> logk, logP, m2;
+  double om = 0, ob = 0, h = 0, ns = 0, A = 1.0, tcmb = 2.7255;
+};
+
+namespace fs2 {
+
+// ---- errors ---------------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void fail(int code, const char* fmt, ...);
+void set_error(int code, const std::string& msg);
+
+#define FS2_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      ::fs2::fail(FS2_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,          \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                \
+  } while (0)
+
+// ---- runtime --------------------------------------------------------------
+void device_init();                 // throws Error(FS2_ERR_DEVICE) without a GPU
+hipStream_t stream();
+void* dmalloc(size_t bytes);        // throws Error(FS2_ERR_MEMORY) on OOM
+void dfree(void* p);
+void sync();
+int loglevel();
+void logf(int level, const char* fmt, ...);
+
+// ---- kernel timing ----------------------------------------------------------
+// Every launch goes through prof_begin/prof_end; both are no-ops unless
+// fs2_profile_enable(1) was called.
+int prof_begin(const char* name, double bytes);
+void prof_end(int token);
+
+template <typename K, typename... Args>
+inline void launch(const char* name, double bytes, K kernel, dim3 grid, dim3 block,
+                   size_t shmem, Args... args) {
+  int tok = prof_begin(name, bytes);
+  hipLaunchKernelGGL(kernel, grid, block, shmem, stream(), args...);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    fail(FS2_ERR_DEVICE, "launch of %s failed: %s", name, hipGetErrorString(e));
+  prof_end(tok);
+}
+
+// ---- FFT (fft.hip) ------------------------------------------------------------
+// Grids are padded in-place r2c layouts: real [n][n][2(n/2+1)] floats, or
+// complex [n][n][n/2+1] float2 in natural [kx][ky][kz] order.
+bool fft_supported(int n);
+void fft_r2c_3d(float* grid, int n);
+void fft_c2r_3d(float* grid, int n);
+// Green x gradient fused into the first inverse pass (x pencils):
+//   dst = IFFT_x( i f2(k) delta_k ),  f2 = f1 k_axis / |k|^2, Nyquist -> +n/2
+// then the y pass and the z c2r pass in place on dst.
+void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f1);
+
+// ---- PM kernels (pm_kernels.hip) -----------------------------------------------
+struct PosArgs {
+  float* x;            // planes [3][stride]
+  int64_t stride;
+  int64_t np;
+};
+void fill_grid(float* grid, int n, float value);
+void wrap_positions(PosArgs p, float boxsize);
+void paint_atomic(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap);
+void readout3(PosArgs p, const float* g0, const float* g1, const float* g2, float* f,
+              int n, float boxsize);
+double grid_sum(const float* grid, int n);   // FP64 sum over the n^3 cells
+void power_spectrum(const float* delta_k, int n, double boxsize, int nbins,
+                    double dk_bin, double* k, double* P, double* nmodes);
+
+// ---- COLA kernels (cola_kernels.hip) --------------------------------------------
+void cola_kick(float* v, const float* f, const float* dx1, const float* dx2,
+               int64_t stride, int64_t np, double m15om, float kf, float q1, float q2);
+void cola_drift(float* x, const float* v, const float* dx1, const float* dx2,
+                int64_t stride, int64_t np, double dt, float da1, float da2);
+void cola_velocity(float* out, const float* v, const float* dx1, const float* dx2,
+                   int64_t stride, int64_t np, float Dv, float D2v);
+void scale_field(float* out, const float* in, int64_t stride, int64_t np, float s);
+
+// ---- LPT kernels (lpt_kernels.hip) --------------------------------------------------
+void white_noise(float* grid, int n, uint64_t seed);
+void lpt_psi_ij(float* dk, float* const out[6], int n, double boxsize,
+                const double* amp /* device, indexed by integer |k|^2 */);
+void lpt_div2(float* out, const float* const psi[6], int n);
+void lpt_psi12(float* psi1_z_inout, const float* div_k, float* const out5[5], int n,
+               double boxsize, const double* amp);
+struct LatticeArgs {
+  float *x, *v, *dx1, *dx2;
+  uint64_t* id;
+  int64_t stride;
+};
+void lpt_lattice(LatticeArgs out, const float* const psi1[3], const float* const psi2[3],
+                 int n, float dx, float offset, float D1, float D2, float Dv, float D2v,
+                 double nmesh3_inv);
+
+// ---- host numerics (host_numerics.cpp) ------------------------------------------------
+namespace host {
+void cosmology_init(double omega_m);
+bool cosmology_ready();
+double omega_m();
+double D_growth(double a);
+double D2_growth(double a, double D);
+double f_growth_rate(double a);
+double hubble(double a);
+double Dv_growth(double a, double D);
+double D2v_growth(double a, double D2);
+double drift_Sq(double ai, double af, double av);
+}  // namespace host
+
+}  // namespace fs2

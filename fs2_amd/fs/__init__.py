@@ -1,0 +1,27 @@
+"""fs: fast cosmological simulation -- PM + COLA on MI355X.
+
+Mirrors py/fs/__init__.py of the reference: importing the package
+initialises communication (one rank here) and exposes the same modules.
+Leapfrog, FoF and the kd-tree are not part of this package.
+"""
+import atexit
+
+from . import _fs
+from . import comm
+from . import cosmology
+from . import lpt
+from . import pm
+from . import cola
+from . import msg
+from .particles import Particles
+from .power import PowerSpectrum
+from .fft import FFT
+
+_fs.comm_mpi_init()
+
+atexit.register(_fs.comm_mpi_finalise)
+
+
+def config_precision():
+    """'single': state, meshes and forces are FP32 (py/py_config.cpp)."""
+    return _fs.config_precision()

@@ -1,0 +1,244 @@
+/*
+ * fs2hip.h -- C ABI of the MI355X (gfx950) PM + COLA hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No
+ * C++ exception and no torch type crosses this boundary.  Every entry point
+ * returns FS2_OK (0) or an FS2_ERR_* code; fs2_last_error() gives the message
+ * of the last failure on the calling thread.
+ *
+ * Each declaration names the fs2 reference interface it replaces (paths are
+ * relative to the fs2 source tree).  The reference keeps PM, LPT and
+ * cosmology state in module globals (py/pm.cpp:22-30, py/lpt.cpp:22-40,
+ * py/cosmology.cpp:15-21); here PM state lives in an fs2_pm handle and the
+ * CPython layer keeps the process-wide one, as py/py_pm.cpp does.
+ *
+ * Device layout: particle fields live in HBM as planes [3][np_alloc] of FP32
+ * (x, v, dx1, dx2, force) plus a u64 id array.  Host views use the
+ * reference's 56-byte AoS record (py/particle.h:12-18) -- see
+ * fs2_particles_get_records.  Meshes are padded in place r2c grids
+ * [nc][nc][2(nc/2+1)] FP32 like py/fft.cpp:19-50.
+ *
+ * All device work is enqueued on one library-owned HIP stream (default
+ * priority); calls that hand data back to the host synchronise it.
+ */
+#ifndef FS2HIP_H
+#define FS2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, mapped by the CPython layer onto the exception types the
+ * reference raises (py/error.h:4-14; py/py_pm.cpp, py/py_power.cpp). */
+enum {
+  FS2_OK = 0,
+  FS2_ERR_RUNTIME = 1,    /* RuntimeError   */
+  FS2_ERR_ASSERTION = 2,  /* AssertionError */
+  FS2_ERR_MEMORY = 3,     /* MemoryError    */
+  FS2_ERR_IO = 4,         /* IOError        */
+  FS2_ERR_VALUE = 5,      /* ValueError     */
+  FS2_ERR_DEVICE = 6,     /* RuntimeError: no GPU, launch or HIP failure */
+  FS2_ERR_INDEX = 7       /* IndexError     */
+};
+
+/* Paint variants for fs2_pm_set_paint_variant. */
+enum {
+  FS2_PAINT_ATOMIC = 0,        /* per-particle FP32 atomics (default)       */
+  FS2_PAINT_SORTED = 1         /* deterministic sort-by-cell segmented sum  */
+};
+
+/* Particle fields for fs2_particles_get_field / set_field. */
+enum {
+  FS2_FIELD_X = 0, FS2_FIELD_DX1 = 1, FS2_FIELD_DX2 = 2,
+  FS2_FIELD_V = 3, FS2_FIELD_FORCE = 4
+};
+
+typedef struct fs2_particles fs2_particles;
+typedef struct fs2_pm fs2_pm;
+typedef struct fs2_grid fs2_grid;
+typedef struct fs2_power fs2_power;
+
+/* ---- library ---------------------------------------------------------- */
+const char* fs2_last_error(void);
+int fs2_version(void);
+/* py/py_msg.cpp set_loglevel: 0 debug .. 4 fatal (py/msg.h:4). */
+int fs2_set_loglevel(int level);
+/* Selects device 0 and creates the library stream.  FS2_ERR_DEVICE when no
+ * GPU is present.  Called implicitly by every entry point that needs it. */
+int fs2_device_init(void);
+int fs2_synchronize(void);
+/* The library stream as a hipStream_t cast to void*. */
+int fs2_get_stream(void** stream);
+
+/* ---- cosmology (FP64, host) ------------------------------------------- */
+/* py/cosmology.cpp:24-30 cosmology_init */
+int fs2_cosmology_init(double omega_m);
+/* py/cosmology.cpp:33-38 cosmology_D_growth */
+int fs2_cosmology_D_growth(double a, double* out);
+/* py/py_cosmology.cpp:29-38: D2(a, D1(a)), py/cosmology.cpp:40-46 */
+int fs2_cosmology_D2_growth(double a, double* out);
+/* py/cosmology.cpp:71-82 cosmology_f_growth_rate */
+int fs2_cosmology_f_growth_rate(double a, double* out);
+/* py/cosmology.cpp:48-56 cosmology_Dv_growth(a, D1(a)) */
+int fs2_cosmology_Dv_growth(double a, double* out);
+/* py/cosmology.cpp:104-108 cosmology_hubble_function */
+int fs2_cosmology_hubble(double a, double* out);
+/* py/cola.cpp:57-66: FP64 kick factor, q1 = D1(a_x), q2 = D2 - D1^2. */
+int fs2_cola_kick_coefficients(double a_v, double a_x, double a_vel,
+                               double* kick_factor, double* q1, double* q2);
+/* py/cola.cpp:102-109: dt = Sq(a_x, a_pos, a_v), da1, da2. */
+int fs2_cola_drift_coefficients(double a_x, double a_v, double a_pos,
+                                double* dt, double* da1, double* da2);
+
+/* ---- linear power spectrum (FP64, host) -------------------------------- */
+/* py/power.cpp:15-40 PowerSpectrum(filename); FS2_ERR_IO on a bad file. */
+int fs2_power_alloc(const char* filename, fs2_power** out);
+/* No reference counterpart: Eisenstein & Hu 1998 no-wiggle P(k) normalised
+ * to sigma8 (top hat R = 8 Mpc/h), k in h/Mpc. */
+int fs2_power_alloc_eh98(double omega_m, double omega_b, double h, double ns,
+                         double sigma8, fs2_power** out);
+void fs2_power_free(fs2_power* ps);
+/* py/py_power.cpp:66-77 _power_n */
+int fs2_power_n(const fs2_power* ps, int* n);
+/* py/py_power.cpp:120-138 _power_i; FS2_ERR_INDEX out of range. */
+int fs2_power_i(const fs2_power* ps, int i, double* k, double* P);
+/* py/power.cpp:50-54 PowerSpectrum::P */
+int fs2_power_P(const fs2_power* ps, double k, double* P);
+/* py/power.cpp:146-171 compute_sigma (tables); top-hat integral (EH98). */
+int fs2_power_sigma(const fs2_power* ps, double R, double* sigma);
+
+/* ---- particles ---------------------------------------------------------- */
+/* py/particle.cpp:13-27 Particles(np_alloc, boxsize) / py_particles_alloc */
+int fs2_particles_alloc(uint64_t np_alloc, double boxsize, fs2_particles** out);
+/* py/py_particles.cpp:157-164 py_particles_free */
+void fs2_particles_free(fs2_particles* p);
+/* py/py_particles.cpp:166-178 _particles_len */
+int fs2_particles_len(const fs2_particles* p, uint64_t* np_local);
+/* py/py_particles.cpp:358-369 _particles_np_total */
+int fs2_particles_np_total(const fs2_particles* p, uint64_t* np_total);
+/* py/particle.cpp:35-41 update_np_total (one rank: np_total = np_local) */
+int fs2_particles_update_np_total(fs2_particles* p);
+int fs2_particles_np_allocated(const fs2_particles* p, uint64_t* np_alloc);
+int fs2_particles_boxsize(const fs2_particles* p, double* boxsize);
+/* py/py_particles.cpp:403-462 _particles_append: n rows of FP64 x,y,z at
+ * x[i*row_stride + c*col_stride] (strides in elements).  Other fields of the
+ * new particles are zero.  FS2_ERR_MEMORY past np_alloc. */
+int fs2_particles_append(fs2_particles* p, const double* x, uint64_t n,
+                         int64_t row_stride, int64_t col_stride);
+/* py/py_particles.cpp:386-401 _particles_clear */
+int fs2_particles_clear(fs2_particles* p);
+/* py/py_particles.cpp:306-355 _particles_x_asarray (x, dx1, dx2) and
+ * _particles_force_asarray: (np_local, 3) FP32 row-major into out. */
+int fs2_particles_get_field(fs2_particles* p, int field, float* out);
+/* Host -> device upload of one field for np_local particles (no reference
+ * counterpart; used to stage test states). */
+int fs2_particles_set_field(fs2_particles* p, int field, const float* in);
+/* py/py_particles.cpp:291-304 _particles_id_asarray */
+int fs2_particles_get_id(fs2_particles* p, uint64_t* out);
+/* py/py_particles.cpp:217-270 _particles_getitem / _slice rows: 56-byte
+ * Particle records (py/particle.h:12-18) for [start, start+count). */
+int fs2_particles_get_records(fs2_particles* p, uint64_t start, uint64_t count,
+                              void* out);
+/* py/py_particles.cpp:371-384 _particles_periodic_wrapup (py/util.h:14-24) */
+int fs2_particles_periodic_wrapup(fs2_particles* p);
+/* Particles::a_x, a_v, a_f (py/particle.h:31) */
+int fs2_particles_get_times(const fs2_particles* p, double* a_x, double* a_v,
+                            double* a_f);
+int fs2_particles_set_times(fs2_particles* p, double a_x, double a_v, double a_f);
+/* Device planes [3][stride] of each field and the id array (for callers
+ * that launch their own work, e.g. autograd).  Synchronises pending wraps. */
+int fs2_particles_device_fields(fs2_particles* p, float** x, float** v,
+                                float** dx1, float** dx2, float** force,
+                                uint64_t** id, uint64_t* stride);
+/* Device-to-device copy of all state of src into dst (same np_alloc). */
+int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src);
+
+/* ---- FFT grids ---------------------------------------------------------- */
+/* py/py_fft.cpp:10-30 _fft_alloc(nc): owning padded real/complex grid */
+int fs2_grid_alloc(int nc, fs2_grid** out);
+void fs2_grid_free(fs2_grid* g);
+int fs2_grid_nc(const fs2_grid* g, int* nc);
+/* py/py_fft.cpp:57-85 _fft_set_test_data: fx[ix,iy,iz] = index + 1 */
+int fs2_grid_set_test_data(fs2_grid* g);
+/* py/py_fft.cpp:87-188 _fft_fx_global_as_array: (nc,nc,nc) FP32 copy */
+int fs2_grid_get_real(fs2_grid* g, float* out);
+int fs2_grid_set_real(fs2_grid* g, const float* in);
+/* Half-complex view nc*nc*(nc/2+1) complex64 in [kx][ky][kz] order. */
+int fs2_grid_get_complex(fs2_grid* g, float* out);
+int fs2_grid_set_complex(fs2_grid* g, const float* in);
+/* py/fft.cpp:80-89 execute_forward (r2c, unnormalised, sign -1) */
+int fs2_grid_r2c(fs2_grid* g);
+/* py/fft.cpp:91-100 execute_inverse (c2r, unnormalised) */
+int fs2_grid_c2r(fs2_grid* g);
+
+/* ---- PM ----------------------------------------------------------------- */
+/* py/pm.cpp:225-268 pm_init: FS2_ERR_RUNTIME for nc_pm <= 1 or boxsize <= 0 */
+int fs2_pm_create(int nc_pm, double pm_factor, double boxsize, fs2_pm** out);
+/* py/pm.cpp:270-274 pm_free */
+void fs2_pm_destroy(fs2_pm* pm);
+int fs2_pm_params(const fs2_pm* pm, int* nc, double* pm_factor, double* boxsize);
+int fs2_pm_set_paint_variant(fs2_pm* pm, int variant);
+/* py/pm_domain.cpp:94-126 pm_domain_send_positions: the periodic wrap;
+ * fused into the next paint, or run on its own before any other use of x. */
+int fs2_pm_send_positions(fs2_pm* pm, fs2_particles* p);
+/* py/pm.cpp:341-355 pm_compute_density; *density is borrowed (valid until
+ * the next call on pm), like the NULL-destructor capsule of py_pm.cpp:55. */
+int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density);
+/* py/pm.cpp:357-390 pm_check_total_density: FP64 sum of delta; returns
+ * FS2_ERR_ASSERTION when |sum| > 10 FLT_EPSILON nc^3. */
+int fs2_pm_check_total_density(fs2_pm* pm, double* sum);
+/* py/pm.cpp:306-338 pm_compute_force: r2c, Green x gradient fused into the
+ * first c2r pass, 3 c2r, CIC readout of all three axes in one pass.
+ * Returns early when a_f == a_x; FS2_ERR_RUNTIME when no density is ready. */
+int fs2_pm_compute_force(fs2_pm* pm, fs2_particles* p);
+/* py/pm_domain.cpp:129-161 pm_domain_get_forces (one rank: no-op) */
+int fs2_pm_get_forces(fs2_pm* pm, fs2_particles* p);
+/* Real-space force mesh of one axis from the last compute_force, (nc)^3
+ * FP32 (no reference counterpart; exposes the fft_pm->fx of pm.cpp:507). */
+int fs2_pm_get_force_mesh(fs2_pm* pm, int axis, float* out);
+/* No reference counterpart: binned P(k) of the current density mesh (FP64
+ * sums).  Must follow compute_density.  Bin b holds b <= |k|/(dk_bin k_f)
+ * < b+1; outputs nbins values each (k in h/Mpc). */
+int fs2_pm_power_spectrum(fs2_pm* pm, int nbins, double dk_bin, double* k,
+                          double* P, double* nmodes);
+
+/* ---- COLA ---------------------------------------------------------------- */
+/* py/cola.cpp:48-92 cola_kick */
+int fs2_cola_kick(fs2_particles* p, double a_vel);
+/* py/cola.cpp:94-127 cola_drift */
+int fs2_cola_drift(fs2_particles* p, double a_pos);
+/* py/cola.cpp:129-148 cola_velocity: (np_local, 3) FP32 into out */
+int fs2_cola_velocity(fs2_particles* p, float* out);
+
+/* ---- 2LPT ---------------------------------------------------------------- */
+/* py/py_lpt.cpp:9-42 _lpt(nc, boxsize, a, seed, ps, kind) with kind one of
+ * "zeldovich", "2lpt", "cola".  The Gaussian field is seeded white noise
+ * (Philox4x32-10 + Box-Muller) instead of N-GenIC's GSL stream. */
+int fs2_lpt(int nc, double boxsize, double a, uint64_t seed, const fs2_power* ps,
+            const char* kind, fs2_particles** out);
+/* py/lpt.cpp:227-230 lpt_set_offset */
+int fs2_lpt_set_offset(double offset);
+/* py/py_lpt.cpp:57-83 _lpt_set_zeldovich_force: f = D1(a) dx1 */
+int fs2_lpt_set_zeldovich_force(fs2_particles* p, double a);
+/* The white-noise field of fs2_lpt for (nc, seed), (nc)^3 FP32 on the host
+ * (no reference counterpart; exposes the field for checks). */
+int fs2_lpt_white_noise(int nc, uint64_t seed, float* out);
+
+/* ---- kernel timing --------------------------------------------------------- */
+/* No reference counterpart (the reference has no timers).  When enabled,
+ * every kernel launch is bracketed by HIP events on the library stream and
+ * accumulated per kernel name with its algorithmic HBM bytes per launch. */
+int fs2_profile_enable(int on);
+int fs2_profile_reset(void);
+int fs2_profile_count(int* n);
+int fs2_profile_get(int i, const char** name, uint64_t* launches,
+                    double* total_ms, double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FS2HIP_H */

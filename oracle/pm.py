@@ -1,0 +1,85 @@
+"""PM force restated from py/pm.cpp:306-508 in FP64 (test oracle only).
+
+Conventions (py/fft.cpp:80-100): the forward r2c is unnormalised with sign -1
+(numpy.fft.rfftn), the inverse c2r is unnormalised (numpy irfftn times n^3).
+numpy's irfftn runs the complex inverse over the two outer axes first and
+the c2r over the last axis last, discarding the imaginary parts of the kz=0
+and kz=n/2 planes -- the same result FFTW's c2r gives on the non-Hermitian
+planes that multiplying by an odd function of k produces.
+
+Green x gradient (pm.cpp:460-504):
+    f1 = -1/n^3 / (2 pi / L),  f2 = f1 * k_axis / |k|^2 (integer k),
+    fk = i f2 delta_k,  k = 0 -> 0,
+with indices <= n/2 mapped to +index (Nyquist is +n/2, pm.cpp:483, 489) and
+kz running over 0..n/2.
+"""
+import numpy as np
+
+from . import cic
+
+
+def int_kvectors_pm(n):
+    """Integer wave numbers along x/y (Nyquist -> +n/2) and z (0..n/2)."""
+    k = np.arange(n)
+    kxy = np.where(k <= n // 2, k, k - n).astype(np.float64)
+    kz = np.arange(n // 2 + 1).astype(np.float64)
+    return kxy, kz
+
+
+def density(x, nc, boxsize, pm_factor):
+    return cic.paint(x, nc, boxsize, pm_factor)
+
+
+def force_meshes(delta, boxsize):
+    """Three real force meshes f_a = irfft(i f2 k_a delta_k) (pm.cpp:460-508)."""
+    n = delta.shape[0]
+    dk = np.fft.rfftn(delta)
+    kxy, kz = int_kvectors_pm(n)
+    KX = kxy[:, None, None]
+    KY = kxy[None, :, None]
+    KZ = kz[None, None, :]
+    k2 = KX**2 + KY**2 + KZ**2
+    k2[0, 0, 0] = 1.0
+    f1 = -1.0 / n**3 / (2.0 * np.pi / boxsize)
+    out = []
+    for K in (KX, KY, KZ):
+        f2 = f1 / k2 * K
+        fk = 1j * f2 * dk
+        fk[0, 0, 0] = 0.0
+        out.append(np.fft.irfftn(fk, s=(n, n, n)) * n**3)
+    return out
+
+
+def force_meshes_fp32(delta, boxsize, workers=1):
+    """The same three meshes in FP32 with scipy.fft's multithreaded
+    pocketfft; the CPU-baseline port of pm.cpp:431-508 (reference Float)."""
+    from scipy import fft as sfft
+    n = delta.shape[0]
+    dk = sfft.rfftn(delta.astype(np.float32, copy=False), workers=workers)
+    kxy, kz = int_kvectors_pm(n)
+    f1 = np.float32(-1.0 / n**3 / (2.0 * np.pi / boxsize))
+    kx = kxy.astype(np.float32)[:, None, None]
+    ky = kxy.astype(np.float32)[None, :, None]
+    kzz = kz.astype(np.float32)[None, None, :]
+    out = []
+    for axis in range(3):
+        fk = np.empty_like(dk)
+        for ix in range(n):   # slab by slab keeps temporaries small
+            k2 = kx[ix] ** 2 + ky[0] ** 2 + kzz[0] ** 2
+            if ix == 0:
+                k2[0, 0] = 1.0
+            K = (kx[ix] + 0 * k2, ky[0] + 0 * k2, kzz[0] + 0 * k2)[axis]
+            f2 = (f1 / k2 * K).astype(np.float32)
+            fk[ix] = 1j * f2 * dk[ix]
+            if ix == 0:
+                fk[0, 0, 0] = 0
+        out.append(sfft.irfftn(fk, s=(n, n, n), workers=workers) * np.float32(n**3))
+        del fk
+    return out
+
+
+def force(x, nc, boxsize, pm_factor):
+    """Forces at (wrapped) positions x: paint -> r2c -> Green -> c2r -> readout."""
+    delta = density(x, nc, boxsize, pm_factor)
+    meshes = force_meshes(delta, boxsize)
+    return cic.readout(x, meshes, boxsize)

@@ -1,0 +1,145 @@
+"""GPU parity: COLA kick/drift, 2LPT initial conditions, a full COLA run and
+the P(k) estimator against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import cola as ocola
+from oracle import cosmology as ocosmo
+from oracle import lpt as olpt
+from oracle import pk as opk
+from oracle import pm as opm
+from oracle import power as opower
+from oracle import rng as orng
+
+pytestmark = pytest.mark.gpu
+
+OM = 0.31
+
+
+def _announce(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def cosmo(fs):
+    fs.cosmology.init(OM)
+    return ocosmo.Cosmology(OM)
+
+
+@pytest.fixture(scope="module")
+def eh(fs):
+    return fs.PowerSpectrum.eisenstein_hu(OM, 0.049, 0.67, 0.96, 0.81)
+
+
+@pytest.fixture(scope="module")
+def eh_oracle():
+    return opower.EH98NoWiggle(OM, 0.049, 0.67, 0.96, 0.81)
+
+
+def test_white_noise_matches_philox_oracle(fs):
+    _announce("white noise")
+    n = 16
+    got = fs.lpt.white_noise(n, 12345).astype(np.float64)
+    ref = orng.white_noise(n, 12345)
+    assert np.abs(got - ref).max() < 1e-6 * np.abs(ref).max()
+    assert abs(got.mean()) < 0.1 and abs(got.std() - 1) < 0.05
+
+
+@pytest.mark.parametrize("kind", ["cola", "zeldovich", "2lpt"])
+def test_lpt_matches_oracle(fs, cosmo, eh, eh_oracle, kind):
+    _announce(f"lpt kind={kind}")
+    n, L, a, seed = 16, 64.0, 0.1, 7
+    p = fs.lpt.init(n, L, a, eh, seed, kind)
+    noise = orng.white_noise(n, seed)
+    ref = olpt.particles(noise, L, a, eh_oracle.P, cosmo, kind=kind)
+    assert np.array_equal(p.id, ref["id"])
+    for name in ("dx1", "dx2", "x"):
+        got = getattr(p, name).astype(np.float64)
+        r = ref[name]
+        scale = np.sqrt(np.mean(ref["dx1"] ** 2)) if name == "x" else np.sqrt(np.mean(r**2))
+        err = np.abs(got - r).max() / scale
+        assert err < 2e-5, (name, err)
+    v = p.v.astype(np.float64)
+    if kind == "cola":
+        assert np.all(v == 0)
+    else:
+        assert np.abs(v - ref["v"]).max() < 2e-5 * np.abs(ref["v"]).max()
+    assert p.times == (a, a, 0.0)
+
+
+def test_lpt_lattice_and_ids_at_a0(fs, cosmo, eh):
+    """test/test_particles_h5.py:57-77: a = 0 gives the exact lattice."""
+    _announce("lpt lattice a=0 offset")
+    n, L = 8, 64.0
+    p = fs.lpt.init(n, L, 0.0, eh, 1, "cola")
+    i = np.arange(n, dtype=np.float32)
+    q = np.stack(np.meshgrid(i, i, i, indexing="ij"), -1).reshape(-1, 3)
+    expect = (q + np.float32(0.5)) * np.float32(L / n)
+    assert np.array_equal(p.x, expect)
+    assert np.array_equal(p.id, np.arange(1, n**3 + 1, dtype=np.uint64))
+
+
+def test_kick_drift_match_oracle(fs, cosmo):
+    _announce("kick drift")
+    rng = np.random.default_rng(3)
+    N, L = 1000, 64.0
+    p = fs.Particles(N, L)
+    p.append(rng.uniform(0, L, (N, 3)))
+    for name in ("v", "dx1", "dx2", "force"):
+        p.set_field(name, rng.standard_normal((N, 3)).astype(np.float32))
+    x0, v0 = p.x.astype(np.float64), p.v.astype(np.float64)
+    dx1, dx2, f = p.dx1.astype(np.float64), p.dx2.astype(np.float64), p.force.astype(np.float64)
+    p.set_times(0.2, 0.15, 0.2)
+    fs.cola.kick(p, 0.25)
+    v1 = ocola.kick(v0, f, dx1, dx2, cosmo, 0.15, 0.2, 0.25)
+    assert np.abs(p.v - v1).max() < 1e-6 * (1 + np.abs(v1).max())
+    assert p.times[1] == 0.25
+    fs.cola.drift(p, 0.3)
+    x1 = ocola.drift(x0, p.v.astype(np.float64), dx1, dx2, cosmo, 0.2, 0.25, 0.3)
+    assert np.abs(p.x - x1).max() < 4e-6 * L
+    assert p.times[0] == 0.3
+
+
+def test_cola_run_matches_oracle(fs, cosmo, eh):
+    """10 COLA steps from 2LPT ICs: final positions within 1e-3 mesh cells."""
+    _announce("cola run 16^3/32^3")
+    n_p, nc, L, a0, a1, nstep = 16, 32, 64.0, 0.1, 1.0, 10
+    p = fs.lpt.init(n_p, L, a0, eh, 11, "cola")
+    x = p.x.astype(np.float64)
+    v = p.v.astype(np.float64)
+    dx1 = p.dx1.astype(np.float64)
+    dx2 = p.dx2.astype(np.float64)
+    fs.pm.init(nc, nc / n_p, L)
+    a_x = a_v = a0
+    for a_vel, a_pos in ocola.schedule(a0, a1, nstep):
+        fs.pm.force(p)
+        fs.cola.kick(p, a_vel)
+        fs.cola.drift(p, a_pos)
+        # oracle step in FP64 on its own trajectory
+        xw = np.mod(x, L)
+        f = opm.force(xw, nc, L, nc / n_p)
+        v = ocola.kick(v, f, dx1, dx2, cosmo, a_v, a_x, a_vel)
+        a_v = a_vel
+        x = ocola.drift(xw, v, dx1, dx2, cosmo, a_x, a_v, a_pos)
+        a_x = a_pos
+    got = p.x.astype(np.float64)
+    d = got - x
+    d -= L * np.round(d / L)
+    cell = L / nc
+    assert np.abs(d).max() / cell < 1e-3, np.abs(d).max() / cell
+
+
+def test_power_spectrum_matches_oracle(fs, cosmo, eh):
+    _announce("power spectrum")
+    n_p, nc, L = 16, 32, 64.0
+    p = fs.lpt.init(n_p, L, 0.5, eh, 5, "cola")
+    fs.pm.init(nc, nc / n_p, L)
+    fs.pm.send_positions(p)
+    g = fs.pm.compute_density(p)
+    delta = g.asarray().astype(np.float64)
+    k, P, nm = fs.pm.power_spectrum(nc=nc)
+    kr, Pr, nmr = opk.power_spectrum(delta, L)
+    assert np.array_equal(nm, nmr)
+    ok = nmr > 0
+    assert np.allclose(k[ok], kr[ok], rtol=1e-12)
+    assert np.allclose(P[ok], Pr[ok], rtol=1e-3)   # P(k) within 0.1%
