@@ -33,7 +33,7 @@ def kvectors_lpt(n, boxsize):
 
 
 def c2r(fk, n):
-    return np.fft.irfftn(fk, s=(n, n, n)) * n**3
+    return np.fft.irfftn(fk, s=(n, n, n), axes=(0, 1, 2)) * n**3
 
 
 def delta_k_from_noise(noise, boxsize, Pfun):

@@ -46,7 +46,7 @@ def force_meshes(delta, boxsize):
         f2 = f1 / k2 * K
         fk = 1j * f2 * dk
         fk[0, 0, 0] = 0.0
-        out.append(np.fft.irfftn(fk, s=(n, n, n)) * n**3)
+        out.append(np.fft.irfftn(fk, s=(n, n, n), axes=(0, 1, 2)) * n**3)
     return out
 
 

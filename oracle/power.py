@@ -81,14 +81,17 @@ def tophat_sigma2(Pfun, R):
     """sigma^2(R) = 1/(2 pi^2) int P(k) W(kR)^2 k^2 dk, integrated in ln k."""
     def integrand(lnk):
         k = np.exp(lnk)
-        x = k * R
-        w = 3.0 * (np.sin(x) - x * np.cos(x)) / x**3 if x > 1e-3 else 1.0 - x * x / 10.0
+        x = np.asarray(k * R, dtype=np.float64)
+        xs = np.maximum(x, 1e-3)
+        w = np.where(x > 1e-3, 3.0 * (np.sin(xs) - xs * np.cos(xs)) / xs**3, 1.0 - x * x / 10.0)
         return Pfun(k) * w * w * k**3
     lo, hi = np.log(1e-6), np.log(1e3)
-    edges = np.linspace(lo, hi, 64)
+    edges = np.linspace(lo, hi, 161)
+    rough = sum(abs(integrate.fixed_quad(integrand, a, b, n=32)[0])
+                for a, b in zip(edges[:-1], edges[1:]))
     tot = 0.0
     for a, b in zip(edges[:-1], edges[1:]):
-        v, _ = integrate.quad(integrand, a, b, epsabs=0.0, epsrel=1e-12, limit=200)
+        v, _ = integrate.quad(integrand, a, b, epsabs=1e-15 * rough, epsrel=1e-12, limit=200)
         tot += v
     return tot / (2.0 * np.pi**2)
 

@@ -42,7 +42,7 @@ def test_c2r_non_hermitian_matches_irfftn(fs, n):
     got = g.asarray()
     # FFTW c2r semantics == numpy irfftn (x/y complex inverse, then c2r on z
     # dropping Im of kz = 0 and kz = n/2), unnormalised
-    ref = np.fft.irfftn(ck.astype(np.complex128), s=(n, n, n)) * n**3
+    ref = np.fft.irfftn(ck.astype(np.complex128), s=(n, n, n), axes=(0, 1, 2)) * n**3
     err = np.abs(got - ref).max() / np.abs(ref).max()
     assert err < 2e-6, err
 
@@ -123,7 +123,9 @@ def test_force_matches_fp64_oracle(fs, n_p, nc):
 
 
 def test_force_plane_wave_closed_form(fs):
-    """delta = A cos(k.x) gives f = -A k/|k|^2 sin(k.x) on the mesh."""
+    """f = grad laplacian^-1 delta (doc/theory/fs.tex): delta = A cos(k.x)
+    gives f = +A k/|k|^2 sin(k.x) on the mesh (pm.cpp:472-501 with
+    fk = i f2 delta_k, f2 = -(L/2pi) k/|k|^2)."""
     _announce("plane wave force mesh")
     nc, L = 32, 32.0
     # particles at cell centres weighted to make delta = A cos(2 pi m x / L):
@@ -135,7 +137,7 @@ def test_force_plane_wave_closed_form(fs):
     delta = A * np.cos(2 * np.pi * m * X / L)
     f = opm.force_meshes(delta, L)
     kphys = 2 * np.pi * m / L
-    expect = -A * kphys / kphys**2 * np.sin(kphys * X)
+    expect = A * kphys / kphys**2 * np.sin(kphys * X)
     assert np.abs(f[0] - expect).max() < 1e-12
     assert np.abs(f[1]).max() < 1e-12 and np.abs(f[2]).max() < 1e-12
     # GPU: same mesh pipeline fed the delta through the density grid
