@@ -35,7 +35,7 @@ HIP_FLAGS = [
     '-Wall', '-Wno-unused-function', '-Wno-unused-variable',
 ]
 
-SOURCES = ['fft.hip', 'pm_kernels.hip', 'cola_kernels.hip', 'lpt_kernels.hip',
+SOURCES = ['fft.hip', 'pm_kernels.hip', 'paint_tiled.hip', 'cola_kernels.hip', 'lpt_kernels.hip',
            'capi.cpp', 'host_numerics.cpp']
 
 

@@ -45,8 +45,9 @@ struct fs2_pm {
   float* force[3] = {nullptr, nullptr, nullptr};
   fs2_grid density;           // borrowed view of mesh
   int status = PM_DONE;
-  int variant = FS2_PAINT_ATOMIC;
+  int variant = FS2_PAINT_SORTED;
   bool mesh_is_k = false;
+  fs2::PaintScratch paint;     // tiled-paint scratch, grown on demand
 };
 
 namespace fs2 {
@@ -213,7 +214,6 @@ void prof_end(int token) {
 // ---- helpers ----------------------------------------------------------------------------------
 namespace {
 
-size_t grid_floats(int nc) { return (size_t)nc * nc * (2 * (nc / 2 + 1)); }
 
 fs2_particles* check_p(fs2_particles* p) {
   if (!p) fail(FS2_ERR_VALUE, "null particles handle");
@@ -262,20 +262,45 @@ fs2_grid* grid_new(int nc) {
 }
 
 void grid_get_real(const fs2_grid* g, float* out) {
-  const int n = g->nc, ncz = 2 * (n / 2 + 1);
+  const int n = g->nc, ncz = grid_rpitch(n);
   FS2_HIP(hipMemcpy2DAsync(out, n * sizeof(float), g->d, ncz * sizeof(float), n * sizeof(float),
                            (size_t)n * n, hipMemcpyDeviceToHost, stream()));
   sync();
 }
 
 void grid_set_real(fs2_grid* g, const float* in) {
-  const int n = g->nc, ncz = 2 * (n / 2 + 1);
+  const int n = g->nc, ncz = grid_rpitch(n);
   FS2_HIP(hipMemcpy2DAsync(g->d, ncz * sizeof(float), in, n * sizeof(float), n * sizeof(float),
                            (size_t)n * n, hipMemcpyHostToDevice, stream()));
   sync();
 }
 
 float lpt_offset = 0.5f;   // py/lpt.cpp:29
+
+void free_paint_scratch(PaintScratch& s) {
+  dfree(s.keys);
+  dfree(s.counts);
+  dfree(s.offsets);
+  dfree(s.cursor);
+  dfree(s.block_sums);
+  dfree(s.sx);
+  s = PaintScratch();
+}
+
+void ensure_paint_scratch(PaintScratch& s, size_t np, size_t nbins) {
+  if (s.keys && s.np >= np && s.nbins == nbins) return;
+  if (g_inited) FS2_HIP(hipStreamSynchronize(g_stream));
+  free_paint_scratch(s);
+  const size_t cap = std::max<size_t>(np, 1);
+  s.keys = (unsigned*)dmalloc(cap * sizeof(unsigned));
+  s.counts = (unsigned*)dmalloc(nbins * sizeof(unsigned));
+  s.offsets = (unsigned*)dmalloc((nbins + 1) * sizeof(unsigned));
+  s.cursor = (unsigned*)dmalloc(nbins * sizeof(unsigned));
+  s.block_sums = (unsigned*)dmalloc(4096 * sizeof(unsigned));
+  s.sx = (float*)dmalloc(3 * cap * sizeof(float));
+  s.np = cap;
+  s.nbins = nbins;
+}
 
 }  // namespace
 }  // namespace fs2
@@ -610,15 +635,19 @@ int fs2_grid_set_real(fs2_grid* g, const float* in) {
 }
 int fs2_grid_get_complex(fs2_grid* g, float* out) {
   return guard([&] {
-    FS2_HIP(hipMemcpyAsync(out, g->d, grid_floats(g->nc) * sizeof(float), hipMemcpyDeviceToHost,
-                           stream()));
+    const int n = g->nc;
+    const size_t w = (size_t)(n / 2 + 1) * 8;
+    FS2_HIP(hipMemcpy2DAsync(out, w, g->d, (size_t)grid_cpitch(n) * 8, w, (size_t)n * n,
+                             hipMemcpyDeviceToHost, stream()));
     sync();
   });
 }
 int fs2_grid_set_complex(fs2_grid* g, const float* in) {
   return guard([&] {
-    FS2_HIP(hipMemcpyAsync(g->d, in, grid_floats(g->nc) * sizeof(float), hipMemcpyHostToDevice,
-                           stream()));
+    const int n = g->nc;
+    const size_t w = (size_t)(n / 2 + 1) * 8;
+    FS2_HIP(hipMemcpy2DAsync(g->d, (size_t)grid_cpitch(n) * 8, in, w, w, (size_t)n * n,
+                             hipMemcpyHostToDevice, stream()));
     sync();
   });
 }
@@ -661,6 +690,7 @@ int fs2_pm_create(int nc_pm, double pm_factor, double boxsize, fs2_pm** out) {
 void fs2_pm_destroy(fs2_pm* pm) {
   if (!pm) return;
   if (g_inited) (void)hipStreamSynchronize(g_stream);
+  free_paint_scratch(pm->paint);
   dfree(pm->mesh);
   for (int a = 0; a < 3; ++a) dfree(pm->force[a]);
   delete pm;
@@ -676,7 +706,7 @@ int fs2_pm_params(const fs2_pm* pm, int* nc, double* pm_factor, double* boxsize)
 
 int fs2_pm_set_paint_variant(fs2_pm* pm, int variant) {
   return guard([&] {
-    if (variant != FS2_PAINT_ATOMIC)
+    if (variant != FS2_PAINT_ATOMIC && variant != FS2_PAINT_SORTED && variant != FS2_PAINT_GLOBAL)
       fail(FS2_ERR_VALUE, "paint variant %d not available", variant);
     pm->variant = variant;
   });
@@ -694,10 +724,16 @@ int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density) {
   return guard([&] {
     check_p(p);
     const int n = pm->nc;
-    // pm.cpp:414-426 clear to -1, then pm.cpp:57-139 (fac = pm_factor^3)
-    fill_grid(pm->mesh, n, -1.0f);
-    const float fac = (float)(pm->pm_factor * pm->pm_factor * pm->pm_factor);
-    paint_atomic(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending);
+    const float fac = (float)(pm->pm_factor * pm->pm_factor * pm->pm_factor);   // pm.cpp:78
+    if (pm->variant == FS2_PAINT_GLOBAL) {
+      // pm.cpp:414-426 clear to -1, then the scatter of pm.cpp:57-139
+      fill_grid(pm->mesh, n, -1.0f);
+      paint_atomic(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending);
+    } else {
+      ensure_paint_scratch(pm->paint, p->np_local, paint_tiled_bins(n));
+      paint_tiled(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending,
+                  pm->variant == FS2_PAINT_SORTED, pm->paint);
+    }
     p->wrap_pending = false;
     pm->mesh_is_k = false;
     pm->status = PM_DENSITY_DONE;

@@ -139,8 +139,21 @@ template <> struct Plan<256>  { static constexpr int T = 16; using R = Radices<1
 template <> struct Plan<512>  { static constexpr int T = 64; using R = Radices<8, 8, 8>; };
 template <> struct Plan<1024> { static constexpr int T = 64; using R = Radices<16, 16, 4>; };
 
-// One Stockham stage over one pencil held in LDS at s[j * SJ].
-template <int N, int T, int SJ, int Ns, int R>
+// LDS layouts of one pencil (element j at s[L::at(j)]).  The padding
+// breaks the power-of-two strides of the Stockham writes (rows R apart in
+// the first stage) across LDS banks.
+template <int P>
+struct RowsLayout {   // [j][P] tiles of strided pencils, one pad row per 16
+  static constexpr int rows(int N) { return N + N / 16; }
+  static __device__ __forceinline__ int at(int j) { return (j + (j >> 4)) * P; }
+};
+struct ContigLayout {   // contiguous pencil, one pad element per 8
+  static constexpr int size(int N) { return N + N / 8 + 1; }
+  static __device__ __forceinline__ int at(int j) { return j + (j >> 3); }
+};
+
+// One Stockham stage over one pencil held in LDS at s[L::at(j)].
+template <int N, int T, class L, int Ns, int R>
 __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* __restrict__ tw) {
   static_assert(N % (R * T) == 0, "plan: stage does not divide evenly");
   constexpr int B = N / (R * T);
@@ -149,7 +162,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
   for (int b = 0; b < B; ++b) {
     const int j = t + b * T;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = s[(j + r * (N / R)) * SJ];
+    for (int r = 0; r < R; ++r) v[b][r] = s[L::at(j + r * (N / R))];
   }
   __syncthreads();
 #pragma unroll
@@ -163,26 +176,26 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
     DFT<R>::run(v[b]);
     const int base = (j - k) * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) s[(base + r * Ns) * SJ] = v[b][r];
+    for (int r = 0; r < R; ++r) s[L::at(base + r * Ns)] = v[b][r];
   }
   __syncthreads();
 }
 
-template <int N, int T, int SJ, int Ns, int R, int... Rest>
+template <int N, int T, class L, int Ns, int R, int... Rest>
 __device__ __forceinline__ void stockham_stages(float2* s, int t, const float2* __restrict__ tw) {
-  stockham_stage<N, T, SJ, Ns, R>(s, t, tw);
-  if constexpr (sizeof...(Rest) > 0) stockham_stages<N, T, SJ, Ns * R, Rest...>(s, t, tw);
+  stockham_stage<N, T, L, Ns, R>(s, t, tw);
+  if constexpr (sizeof...(Rest) > 0) stockham_stages<N, T, L, Ns * R, Rest...>(s, t, tw);
 }
 
-template <int N, int T, int SJ, int... Rs>
+template <int N, int T, class L, int... Rs>
 __device__ __forceinline__ void run_plan(float2* s, int t, const float2* __restrict__ tw, Radices<Rs...>) {
-  stockham_stages<N, T, SJ, 1, Rs...>(s, t, tw);
+  stockham_stages<N, T, L, 1, Rs...>(s, t, tw);
 }
 
-// Forward FFT of length N of the pencil s[j*SJ], j < N, by the T threads t.
-template <int N, int SJ>
+// Forward FFT of length N of the pencil s[L::at(j)], j < N, by the T threads t.
+template <int N, class L>
 __device__ __forceinline__ void fft_lds(float2* s, int t, const float2* __restrict__ tw) {
-  run_plan<N, Plan<N>::T, SJ>(s, t, tw, typename Plan<N>::R{});
+  run_plan<N, Plan<N>::T, L>(s, t, tw, typename Plan<N>::R{});
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that
@@ -221,7 +234,8 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
              int64_t outer_stride, int nchunks, int nblocks, const float2* __restrict__ tw,
              GreenArg g) {
   constexpr int T = Plan<N>::T;
-  __shared__ float2 s[N * P];
+  using L = RowsLayout<P>;
+  __shared__ float2 s[L::rows(N) * P];
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
   const int bid = xcd_remap(blockIdx.x, nblocks);
@@ -236,14 +250,14 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
     if (valid) v = src[base + (int64_t)j * elem_stride];
     if constexpr (GREEN) v = green_mul(v, j, o, c, g);
     if constexpr (INV) v.y = -v.y;
-    s[j * P + p] = v;
+    s[L::at(j) + p] = v;
   }
   __syncthreads();
-  fft_lds<N, P>(s + p, t, tw);
+  fft_lds<N, L>(s + p, t, tw);
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
     const int j = t + i * T;
-    float2 v = s[j * P + p];
+    float2 v = s[L::at(j) + p];
     if constexpr (INV) v.y = -v.y;
     if (valid) dst[base + (int64_t)j * elem_stride] = v;
   }
@@ -255,36 +269,37 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
 //   W = e^{-2 pi i/n}, k = 0..M.
 template <int M, int P>
 __global__ void __launch_bounds__(Plan<M>::T * P)
-k_fft_z_r2c(float2* data, int64_t nrows, const float2* __restrict__ twM,
+k_fft_z_r2c(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ twM,
             const float2* __restrict__ twN) {
   constexpr int T = Plan<M>::T;
-  constexpr int LS = M + 1;
+  using L = ContigLayout;
+  constexpr int LS = L::size(M);
   __shared__ float2 s[P * LS];
   const int tid = threadIdx.x;
   const int t = tid % T, p = tid / T;
   const int64_t row = (int64_t)blockIdx.x * P + p;
   const bool valid = row < nrows;
-  float2* r = data + row * (M + 1);
+  float2* r = data + row * cpitch;
   float2* sp = s + p * LS;
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int m = t + i * T;
-    sp[m] = valid ? r[m] : make_float2(0.f, 0.f);
+    sp[L::at(m)] = valid ? r[m] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<M, 1>(sp, t, twM);
+  fft_lds<M, L>(sp, t, twM);
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
-    const float2 A = sp[k];
-    const float2 Bc = cconj(sp[k == 0 ? 0 : M - k]);
+    const float2 A = sp[L::at(k)];
+    const float2 Bc = cconj(sp[L::at(k == 0 ? 0 : M - k)]);
     const float2 E = cscale(cadd(A, Bc), 0.5f);
     const float2 O = cscale(csub(A, Bc), 0.5f);
     const float2 X = cadd(E, mul_negi(cmul(twN[k], O)));
     if (valid) r[k] = X;
   }
   if (t == 0 && valid) {
-    const float2 A = sp[0];
+    const float2 A = sp[L::at(0)];
     r[M] = make_float2(A.x - A.y, 0.f);
   }
 }
@@ -294,30 +309,31 @@ k_fft_z_r2c(float2* data, int64_t nrows, const float2* __restrict__ twM,
 // O' = (X[k] - conj X[M-k]) W^{-k};  z = IDFT_M(Z) = x_even + i x_odd.
 template <int M, int P>
 __global__ void __launch_bounds__(Plan<M>::T * P)
-k_fft_z_c2r(float2* data, int64_t nrows, const float2* __restrict__ twM,
+k_fft_z_c2r(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ twM,
             const float2* __restrict__ twN) {
   constexpr int T = Plan<M>::T;
-  constexpr int LS = M + 1;
+  using L = ContigLayout;
+  constexpr int LS = L::size(M);
   __shared__ float2 s[P * LS];
   const int tid = threadIdx.x;
   const int t = tid % T, p = tid / T;
   const int64_t row = (int64_t)blockIdx.x * P + p;
   const bool valid = row < nrows;
-  float2* r = data + row * (M + 1);
+  float2* r = data + row * cpitch;
   float2* sp = s + p * LS;
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int m = t + i * T;
-    sp[m] = valid ? r[m] : make_float2(0.f, 0.f);
+    sp[L::at(m)] = valid ? r[m] : make_float2(0.f, 0.f);
   }
-  if (t == 0) sp[M] = valid ? r[M] : make_float2(0.f, 0.f);
+  if (t == 0) sp[L::at(M)] = valid ? r[M] : make_float2(0.f, 0.f);
   __syncthreads();
   float2 z[M / T];
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
-    float2 A = sp[k];
-    float2 B = sp[M - k];
+    float2 A = sp[L::at(k)];
+    float2 B = sp[L::at(M - k)];
     if (k == 0) {
       A.y = 0.f;
       B.y = 0.f;
@@ -329,13 +345,13 @@ k_fft_z_c2r(float2* data, int64_t nrows, const float2* __restrict__ twM,
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < M / T; ++i) sp[t + i * T] = z[i];
+  for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = z[i];
   __syncthreads();
-  fft_lds<M, 1>(sp, t, twM);
+  fft_lds<M, L>(sp, t, twM);
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int m = t + i * T;
-    if (valid) r[m] = cconj(sp[m]);
+    if (valid) r[m] = cconj(sp[L::at(m)]);
   }
 }
 
@@ -359,7 +375,7 @@ const float2* twiddles(int N) {
 
 // Columns per block for the strided passes, and rows per block for the z pass.
 template <int N> constexpr int pencil_cols() {
-  return (N * 16 * 8 <= 65536 && Plan<N>::T * 16 <= 1024) ? 16 : 8;
+  return ((N + N / 16) * 16 * 8 <= 81920 && Plan<N>::T * 16 <= 1024) ? 16 : 8;
 }
 template <int M> constexpr int zrows() {
   return Plan<M>::T >= 256 ? 1 : 256 / Plan<M>::T;
@@ -389,18 +405,20 @@ struct Impl {
     const int64_t nrows = (int64_t)n * n;
     const double bytes = (double)n * n * n * 4.0 + grid_bytes();
     launch("fft_z_r2c", bytes, k_fft_z_r2c<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
-           dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, twiddles(M), twiddles(N));
+           dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
+           twiddles(N));
   }
 
   static void z_c2r(float* grid) {
     const int64_t nrows = (int64_t)n * n;
     const double bytes = (double)n * n * n * 4.0 + grid_bytes();
     launch("fft_z_c2r", bytes, k_fft_z_c2r<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
-           dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, twiddles(M), twiddles(N));
+           dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
+           twiddles(N));
   }
 
   static void r2c(float* grid) {
-    const int64_t nh = n / 2 + 1;
+    const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)grid;
     GreenArg g{0, n, 0.f};
     z_r2c(grid);
@@ -409,7 +427,7 @@ struct Impl {
   }
 
   static void c2r(float* grid) {
-    const int64_t nh = n / 2 + 1;
+    const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)grid;
     GreenArg g{0, n, 0.f};
     pencil<true, false>("fft_x_inv", c, c, (int64_t)n * nh, nh, g);
@@ -418,7 +436,7 @@ struct Impl {
   }
 
   static void green_c2r(const float* dk, float* dst, int axis, float f1) {
-    const int64_t nh = n / 2 + 1;
+    const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)dst;
     GreenArg g{axis, n, f1};
     pencil<true, true>("fft_x_inv_green", (const float2*)dk, c, (int64_t)n * nh, nh, g);

@@ -61,9 +61,16 @@ inline void launch(const char* name, double bytes, K kernel, dim3 grid, dim3 blo
   prof_end(tok);
 }
 
+// ---- mesh layout -----------------------------------------------------------
+// In-place r2c grids: n*n rows; a row holds n reals or n/2+1 complex values,
+// and rows start every grid_cpitch(n) float2 = grid_rpitch(n) floats, a
+// multiple of 128 bytes so no row segment straddles a cache line.
+inline int grid_cpitch(int n) { return ((n / 2 + 1) + 15) & ~15; }
+inline int grid_rpitch(int n) { return 2 * grid_cpitch(n); }
+inline size_t grid_floats(int n) { return (size_t)n * n * grid_rpitch(n); }
+
 // ---- FFT (fft.hip) ------------------------------------------------------------
-// Grids are padded in-place r2c layouts: real [n][n][2(n/2+1)] floats, or
-// complex [n][n][n/2+1] float2 in natural [kx][ky][kz] order.
+// Complex values are in natural [kx][ky][kz] order (kz = 0..n/2).
 bool fft_supported(int n);
 void fft_r2c_3d(float* grid, int n);
 void fft_c2r_3d(float* grid, int n);
@@ -84,6 +91,21 @@ void paint_atomic(PosArgs p, float* grid, int n, float boxsize, float fac, bool 
 void readout3(PosArgs p, const float* g0, const float* g1, const float* g2, float* f,
               int n, float boxsize);
 double grid_sum(const float* grid, int n);   // FP64 sum over the n^3 cells
+
+// Tiled paint (paint_tiled.hip): device scratch sized for np particles and
+// nbins = paint_tiled_bins(n) bins.
+struct PaintScratch {
+  unsigned* keys = nullptr;      // [np]
+  unsigned* counts = nullptr;    // [nbins]
+  unsigned* offsets = nullptr;   // [nbins + 1]
+  unsigned* cursor = nullptr;    // [nbins]
+  unsigned* block_sums = nullptr;   // [4096] scan block totals
+  float* sx = nullptr;           // [3][np] positions in bin order
+  size_t np = 0, nbins = 0;
+};
+size_t paint_tiled_bins(int n);
+void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
+                 bool deterministic, const PaintScratch& s);
 void power_spectrum(const float* delta_k, int n, double boxsize, int nbins,
                     double dk_bin, double* k, double* P, double* nmodes);
 

@@ -88,12 +88,13 @@ __device__ __forceinline__ Mode mode_of(int64_t idx, int n, double dk) {
 // Psi_ij = i k_j Psi1_i for ij = 11,12,13,22,23,33 (lpt.cpp:486-502).
 __global__ void k_psi_ij(float2* __restrict__ dk, float2* __restrict__ o0, float2* __restrict__ o1,
                          float2* __restrict__ o2, float2* __restrict__ o3, float2* __restrict__ o4,
-                         float2* __restrict__ o5, int n, double kf,
+                         float2* __restrict__ o5, int n, int cpitch, double kf,
                          const double* __restrict__ amp) {
   const int64_t total = (int64_t)n * n * (n / 2 + 1);
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= total) return;
-  const Mode m = mode_of(idx, n, kf);
+  const int64_t mi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (mi >= total) return;
+  const Mode m = mode_of(mi, n, kf);
+  const int64_t idx = ((int64_t)m.ix * n + m.iy) * cpitch + m.iz;
   float2 d = make_float2(0.f, 0.f);
   if (!m.nyq && m.k2i > 0) {
     const float2 w = dk[idx];
@@ -134,11 +135,13 @@ __global__ void k_div2(float* out, const float* p0, const float* p1, const float
 // delta_k (in/out) receives Psi1_z; out: Psi1_x, Psi1_y, Psi2_x, Psi2_y, Psi2_z.
 __global__ void k_psi12(float2* __restrict__ delta, const float2* __restrict__ divk,
                         float2* __restrict__ q0, float2* __restrict__ q1, float2* __restrict__ q2,
-                        float2* __restrict__ q3, float2* __restrict__ q4, int n, double kf) {
+                        float2* __restrict__ q3, float2* __restrict__ q4, int n, int cpitch,
+                        double kf) {
   const int64_t total = (int64_t)n * n * (n / 2 + 1);
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= total) return;
-  const Mode m = mode_of(idx, n, kf);
+  const int64_t mi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (mi >= total) return;
+  const Mode m = mode_of(mi, n, kf);
+  const int64_t idx = ((int64_t)m.ix * n + m.iy) * cpitch + m.iz;
   const float2 d = delta[idx];
   const float2 D = divk[idx];
   float2 p1[3], p2[3];
@@ -194,7 +197,7 @@ void white_noise(float* grid, int n, uint64_t seed) {
   if (n % 4 != 0) fail(FS2_ERR_VALUE, "white noise needs nc divisible by 4 (nc = %d)", n);
   const int64_t ncall = (int64_t)n * n * n / 4;
   launch("white_noise", (double)n * n * n * 4.0, k_white_noise, dim3(blocks_for(ncall)),
-         dim3(kBlock), 0, grid, n, 2 * (n / 2 + 1), ncall, (uint32_t)(seed & 0xffffffffu),
+         dim3(kBlock), 0, grid, n, grid_rpitch(n), ncall, (uint32_t)(seed & 0xffffffffu),
          (uint32_t)(seed >> 32));
 }
 
@@ -202,13 +205,13 @@ void lpt_psi_ij(float* dk, float* const out[6], int n, double boxsize, const dou
   const int64_t total = (int64_t)n * n * (n / 2 + 1);
   launch("lpt_psi_ij", (double)total * 8.0 * 8.0, k_psi_ij, dim3(blocks_for(total)), dim3(kBlock),
          0, (float2*)dk, (float2*)out[0], (float2*)out[1], (float2*)out[2], (float2*)out[3],
-         (float2*)out[4], (float2*)out[5], n, 2.0 * M_PI / boxsize, amp);
+         (float2*)out[4], (float2*)out[5], n, grid_cpitch(n), 2.0 * M_PI / boxsize, amp);
 }
 
 void lpt_div2(float* out, const float* const psi[6], int n) {
   const int64_t cells = (int64_t)n * n * n;
   launch("lpt_div2", (double)cells * 28.0, k_div2, dim3(blocks_for(cells)), dim3(kBlock), 0, out,
-         psi[0], psi[1], psi[2], psi[3], psi[4], psi[5], n, 2 * (n / 2 + 1));
+         psi[0], psi[1], psi[2], psi[3], psi[4], psi[5], n, grid_rpitch(n));
 }
 
 void lpt_psi12(float* psi1_z_inout, const float* div_k, float* const out5[5], int n,
@@ -217,7 +220,8 @@ void lpt_psi12(float* psi1_z_inout, const float* div_k, float* const out5[5], in
   const int64_t total = (int64_t)n * n * (n / 2 + 1);
   launch("lpt_psi12", (double)total * 8.0 * 8.0, k_psi12, dim3(blocks_for(total)), dim3(kBlock), 0,
          (float2*)psi1_z_inout, (const float2*)div_k, (float2*)out5[0], (float2*)out5[1],
-         (float2*)out5[2], (float2*)out5[3], (float2*)out5[4], n, 2.0 * M_PI / boxsize);
+         (float2*)out5[2], (float2*)out5[3], (float2*)out5[4], n, grid_cpitch(n),
+         2.0 * M_PI / boxsize);
 }
 
 void lpt_lattice(LatticeArgs out, const float* const psi1[3], const float* const psi2[3], int n,
@@ -225,7 +229,7 @@ void lpt_lattice(LatticeArgs out, const float* const psi1[3], const float* const
                  double nmesh3_inv) {
   const int64_t np = (int64_t)n * n * n;
   launch("lpt_lattice", (double)np * (24.0 + 56.0), k_lattice, dim3(blocks_for(np)), dim3(kBlock),
-         0, out, psi1[0], psi1[1], psi1[2], psi2[0], psi2[1], psi2[2], n, 2 * (n / 2 + 1), dx,
+         0, out, psi1[0], psi1[1], psi1[2], psi2[0], psi2[1], psi2[2], n, grid_rpitch(n), dx,
          offset, D1, D2, Dv, D2v, nmesh3_inv);
 }
 

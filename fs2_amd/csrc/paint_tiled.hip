@@ -1,0 +1,413 @@
+// Tiled CIC mass assignment for gfx950 (replaces the scatter loop of
+// py/pm.cpp:57-139 plus the clear of pm.cpp:414-426).
+//
+// Owner-computes: the mesh is cut into tiles of TX x TY x TZ cells; every
+// cell is written once, by the workgroup of the tile that owns it, with a
+// plain store of (sum of CIC contributions) - 1.  No global float atomics,
+// no separate clear pass.
+//
+//   K1 bin    : wrap x (pm_domain.cpp:115 fused), lower CIC cell of each
+//               particle, key = tile * 8 + spill class; the class bits say
+//               whether the lower cell is on the last x / y / z layer of its
+//               tile, i.e. whether the +1 corner lands in the next tile.
+//               Bin counts use wavefront-aggregated atomics (one atomic per
+//               distinct key per wavefront).
+//   K2 scan   : exclusive prefix sum of the counts -> bin offsets.
+//   K3 scatter: positions copied into bin order (wavefront-aggregated
+//               cursor atomics).
+//   K4 tile   : per tile, LDS accumulation of every stencil corner that
+//               lands in the tile, from the tile's own particles and from the
+//               spilling classes of its lower neighbours; then one coalesced
+//               store pass.  FP32 LDS atomics (fast variant) or int64
+//               fixed-point LDS sums (deterministic variant: integer addition
+//               is associative, so the result is bitwise reproducible).
+//
+// CIC indices and weights follow pm.cpp:85-133 in FP32 with contraction
+// disabled, exactly as the per-particle atomic kernel does.
+#include <algorithm>
+#include <cmath>
+
+#include "fs2_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace fs2 {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr unsigned kNoKey = 0xffffffffu;
+constexpr double kFix = 4294967296.0;   // 2^32 fixed-point scale
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__device__ __forceinline__ float wrap1(float x, float L) {
+  if (x < 0.f) x += L;
+  if (x >= L) x -= L;
+  return x;
+}
+
+struct Cic1 {
+  int i0, i1;
+  float w0, w1;
+};
+__device__ __forceinline__ Cic1 cic1(float x, float dx_inv, int n) {
+  Cic1 c;
+  const float X = x * dx_inv;
+  int i0 = (int)X;
+  c.w1 = X - (float)i0;
+  c.w0 = 1.f - c.w1;
+  if (i0 >= n) i0 = 0;
+  if ((unsigned)i0 >= (unsigned)n) i0 = ((i0 % n) + n) % n;
+  int i1 = i0 + 1;
+  if (i1 >= n) i1 -= n;
+  c.i0 = i0;
+  c.i1 = i1;
+  return c;
+}
+
+// Tile extents are powers of two for every supported mesh size, so tile
+// and cell coordinates come from shifts and masks.
+struct TileGeom {
+  int n, ncz;
+  int tx, ty, tz;      // tile extent in cells
+  int lx, ly, lz;      // log2 of the extents
+  int ntx, nty, ntz;   // tiles per axis
+  float dx_inv;
+};
+
+__device__ __forceinline__ unsigned key_of(const TileGeom& g, int ix, int iy, int iz) {
+  const unsigned tile = ((unsigned)(ix >> g.lx) * g.nty + (unsigned)(iy >> g.ly)) * g.ntz +
+                        (unsigned)(iz >> g.lz);
+  const unsigned cls = (((ix & (g.tx - 1)) == g.tx - 1) ? 4u : 0u) |
+                       (((iy & (g.ty - 1)) == g.ty - 1) ? 2u : 0u) |
+                       (((iz & (g.tz - 1)) == g.tz - 1) ? 1u : 0u);
+  return tile * 8u + cls;
+}
+
+// Wavefront-aggregated atomicAdd of 1 per lane to ctr[key]; returns the
+// slot of this lane (old value + rank among lanes with the same key).
+__device__ __forceinline__ unsigned wave_aggregated_inc(unsigned* ctr, unsigned key, bool valid) {
+  const int lane = __lane_id();
+  uint64_t active = __ballot(valid);
+  unsigned slot = 0;
+  while (active) {
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const unsigned lk = __shfl(key, leader);
+    const uint64_t same = __ballot(valid && key == lk) & active;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(ctr + lk, (unsigned)__popcll(same));
+    base = __shfl(base, leader);
+    if ((same >> lane) & 1ull) slot = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+    active &= ~same;
+  }
+  return slot;
+}
+
+__global__ void k_fill_u32(unsigned* __restrict__ a, int64_t n, unsigned v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L, int do_wrap,
+                      TileGeom g, unsigned* __restrict__ keys, unsigned* __restrict__ counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < np;
+  unsigned key = kNoKey;
+  if (valid) {
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      p[a] = x[i + a * stride];
+      if (do_wrap) {
+        const float w = wrap1(p[a], L);
+        if (w != p[a]) x[i + a * stride] = w;
+        p[a] = w;
+      }
+    }
+    const Cic1 cx = cic1(p[0], g.dx_inv, g.n);
+    const Cic1 cy = cic1(p[1], g.dx_inv, g.n);
+    const Cic1 cz = cic1(p[2], g.dx_inv, g.n);
+    key = key_of(g, cx.i0, cy.i0, cz.i0);
+    keys[i] = key;
+  }
+  wave_aggregated_inc(counts, key, valid);
+}
+
+// Exclusive scan of the bin counts in three launches: per-block scans of
+// kScanChunk values, a one-block scan of the block totals, and the add-back
+// that writes offsets and cursors.
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 4;
+constexpr int kScanChunk = kScanThreads * kScanPer;
+
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* lds, unsigned* total) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const unsigned u = t >= off ? lds[t - off] : 0u;
+    __syncthreads();
+    lds[t] += u;
+    __syncthreads();
+  }
+  const unsigned incl = lds[t];
+  if (total) *total = lds[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_blocks(const unsigned* __restrict__ counts, int64_t n, unsigned* __restrict__ partial,
+              unsigned* __restrict__ block_sums) {
+  __shared__ unsigned lds[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  unsigned c[kScanPer];
+  unsigned s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    c[i] = base + i < n ? counts[base + i] : 0u;
+    s += c[i];
+  }
+  unsigned total;
+  unsigned run = block_exclusive_scan(s, lds, &total);
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    if (base + i < n) partial[base + i] = run;
+    run += c[i];
+  }
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of nb block totals (nb <= kScanChunk), in place;
+// offsets[n] receives the grand total.
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_sums(unsigned* __restrict__ block_sums, int nb, unsigned* __restrict__ offsets, int64_t n) {
+  __shared__ unsigned lds[kScanThreads];
+  const int base = threadIdx.x * kScanPer;
+  unsigned c[kScanPer];
+  unsigned s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    c[i] = base + i < nb ? block_sums[base + i] : 0u;
+    s += c[i];
+  }
+  unsigned total;
+  unsigned run = block_exclusive_scan(s, lds, &total);
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    if (base + i < nb) block_sums[base + i] = run;
+    run += c[i];
+  }
+  if (threadIdx.x == 0) offsets[n] = total;
+}
+
+// partial and cursor may be the same buffer (same index read then written).
+__global__ void k_scan_add(const unsigned* partial, const unsigned* __restrict__ block_sums,
+                           int64_t n, unsigned* __restrict__ offsets, unsigned* cursor) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned v = partial[i] + block_sums[i / kScanChunk];
+  offsets[i] = v;
+  cursor[i] = v;
+}
+
+__global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t np,
+                          const unsigned* __restrict__ keys, unsigned* __restrict__ cursor,
+                          float* __restrict__ sx, int64_t sstride) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < np;
+  const unsigned key = valid ? keys[i] : kNoKey;
+  const unsigned slot = wave_aggregated_inc(cursor, key, valid);
+  if (valid) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sx[slot + a * sstride] = x[i + a * stride];
+  }
+}
+
+// K4: one workgroup per tile.  ACC is float (LDS float atomics) or
+// unsigned long long (int64 fixed point, deterministic).
+template <typename ACC>
+__global__ void __launch_bounds__(kBlock)
+k_tile_paint(const float* __restrict__ sx, int64_t sstride, const unsigned* __restrict__ offsets,
+             TileGeom g, float fac, float* __restrict__ grid) {
+  extern __shared__ unsigned char smem[];
+  ACC* acc = (ACC*)smem;
+  const int tcells = g.tx * g.ty * g.tz;
+  for (int c = threadIdx.x; c < tcells; c += blockDim.x) acc[c] = (ACC)0;
+  __syncthreads();
+
+  const int tile = blockIdx.x;
+  const int Tz = tile % g.ntz;
+  const int Ty = (tile / g.ntz) % g.nty;
+  const int Tx = tile / (g.ntz * g.nty);
+  const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
+
+  // Source tiles S = T - d, d in {0,1}^3 (periodic).  A particle of S
+  // reaches T only by spilling along every axis where S != T, so S
+  // contributes the classes c with (c & need) == need.  All 8 x 9 bin
+  // offsets are loaded at once; the needed class runs are merged into at
+  // most 15 contiguous ranges, which all threads then stride over.
+  __shared__ unsigned offs[8][9];
+  __shared__ unsigned rb[16], rpre[17];
+  __shared__ int nranges;
+  if (threadIdx.x < 72) {
+    const int d = threadIdx.x / 9, cl = threadIdx.x % 9;
+    const int Sx = (Tx - ((d >> 2) & 1) + g.ntx) % g.ntx;
+    const int Sy = (Ty - ((d >> 1) & 1) + g.nty) % g.nty;
+    const int Sz = (Tz - (d & 1) + g.ntz) % g.ntz;
+    offs[d][cl] = offsets[(unsigned)((Sx * g.nty + Sy) * g.ntz + Sz) * 8u + cl];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int S_of[8];
+    int nr = 0;
+    unsigned total = 0;
+    for (int d = 0; d < 8; ++d) {
+      const int Sx = (Tx - ((d >> 2) & 1) + g.ntx) % g.ntx;
+      const int Sy = (Ty - ((d >> 1) & 1) + g.nty) % g.nty;
+      const int Sz = (Tz - (d & 1) + g.ntz) % g.ntz;
+      S_of[d] = (Sx * g.nty + Sy) * g.ntz + Sz;
+      bool seen = false;
+      for (int e = 0; e < d; ++e) seen |= (S_of[e] == S_of[d]);
+      if (seen) continue;
+      const unsigned need = (Sx != Tx ? 4u : 0u) | (Sy != Ty ? 2u : 0u) | (Sz != Tz ? 1u : 0u);
+      for (unsigned cl = 0; cl < 8; ++cl) {
+        if ((cl & need) != need) continue;
+        const unsigned b = offs[d][cl], e = offs[d][cl + 1];
+        if (e <= b) continue;
+        if (nr > 0 && rb[nr - 1] + (rpre[nr] - rpre[nr - 1]) == b) {
+          rpre[nr] += e - b;   // extends the previous run
+        } else {
+          rb[nr] = b;
+          rpre[nr] = total;
+          rpre[nr + 1] = total + (e - b);
+          ++nr;
+        }
+        total = rpre[nr];
+      }
+    }
+    if (nr == 0) rpre[0] = 0;
+    nranges = nr;
+  }
+  __syncthreads();
+  const int nr = nranges;
+  const unsigned total = rpre[nr];
+  int r = 0;
+  for (unsigned v = threadIdx.x; v < total; v += blockDim.x) {
+    while (v >= rpre[r + 1]) ++r;
+    const unsigned j = rb[r] + (v - rpre[r]);
+    const Cic1 cx = cic1(sx[j], g.dx_inv, g.n);
+    const Cic1 cy = cic1(sx[j + sstride], g.dx_inv, g.n);
+    const Cic1 cz = cic1(sx[j + 2 * sstride], g.dx_inv, g.n);
+    const int lx[2] = {cx.i0 - x0, cx.i1 - x0};
+    const int ly[2] = {cy.i0 - y0, cy.i1 - y0};
+    const int lz[2] = {cz.i0 - z0, cz.i1 - z0};
+    const float wx[2] = {cx.w0, cx.w1};
+    const float wy[2] = {cy.w0, cy.w1};
+    const float wz[2] = {cz.w0, cz.w1};
+#pragma unroll
+    for (int ex = 0; ex < 2; ++ex) {
+      if ((unsigned)lx[ex] >= (unsigned)g.tx) continue;
+#pragma unroll
+      for (int ey = 0; ey < 2; ++ey) {
+        if ((unsigned)ly[ey] >= (unsigned)g.ty) continue;
+#pragma unroll
+        for (int ez = 0; ez < 2; ++ez) {
+          if ((unsigned)lz[ez] >= (unsigned)g.tz) continue;
+          // fac*wx*wy*wz, left to right (pm.cpp:122-132)
+          const float w = fac * wx[ex] * wy[ey] * wz[ez];
+          const int li = (((lx[ex] << g.ly) + ly[ey]) << g.lz) + lz[ez];
+          if constexpr (sizeof(ACC) == 4) {
+            atomicAdd(&acc[li], w);
+          } else {
+            atomicAdd(&acc[li], (ACC)(long long)llrint((double)w * kFix));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < tcells; c += blockDim.x) {
+    const int lz = c & (g.tz - 1);
+    const int ly = (c >> g.lz) & (g.ty - 1);
+    const int lx = c >> (g.lz + g.ly);
+    float v;
+    if constexpr (sizeof(ACC) == 4) {
+      v = acc[c];
+    } else {
+      v = (float)((double)(long long)acc[c] * (1.0 / kFix));
+    }
+    grid[((int64_t)(x0 + lx) * g.n + (y0 + ly)) * g.ncz + (z0 + lz)] = v - 1.0f;
+  }
+}
+
+TileGeom make_geom(int n, float boxsize) {
+  TileGeom g;
+  g.n = n;
+  g.ncz = grid_rpitch(n);
+  // 8 x 16 x 32 cells: 16 KB of FP32 LDS, 128-byte rows along z
+  g.tx = n % 8 == 0 ? 8 : n;
+  g.ty = n % 16 == 0 ? 16 : n;
+  g.tz = n % 32 == 0 ? 32 : (n % 16 == 0 ? 16 : n);
+  g.lx = __builtin_ctz((unsigned)g.tx);
+  g.ly = __builtin_ctz((unsigned)g.ty);
+  g.lz = __builtin_ctz((unsigned)g.tz);
+  if ((g.tx & (g.tx - 1)) || (g.ty & (g.ty - 1)) || (g.tz & (g.tz - 1)))
+    fail(FS2_ERR_VALUE, "paint tiles need power-of-two extents (n = %d)", n);
+  g.ntx = n / g.tx;
+  g.nty = n / g.ty;
+  g.ntz = n / g.tz;
+  g.dx_inv = (float)n / boxsize;   // pm.cpp:77
+  return g;
+}
+
+}  // namespace
+
+size_t paint_tiled_bins(int n) {
+  const TileGeom g = make_geom(n, 1.0f);
+  return (size_t)g.ntx * g.nty * g.ntz * 8;
+}
+
+void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
+                 bool deterministic, const PaintScratch& s) {
+  const TileGeom g = make_geom(n, boxsize);
+  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
+  const int64_t nbins = ntiles * 8;
+  if ((size_t)nbins > s.nbins || (size_t)p.np > s.np)
+    fail(FS2_ERR_RUNTIME, "paint scratch too small (%lld bins, %lld particles)",
+         (long long)nbins, (long long)p.np);
+  launch("paint_clear_bins", (double)nbins * 4.0, k_fill_u32, dim3(blocks_for(nbins)),
+         dim3(kBlock), 0, s.counts, nbins, 0u);
+  if (p.np > 0) {
+    launch("paint_bin", (double)p.np * 16.0, k_bin, dim3(blocks_for(p.np)), dim3(kBlock), 0, p.x,
+           p.stride, p.np, boxsize, wrap ? 1 : 0, g, s.keys, s.counts);
+  }
+  const int nb = (int)((nbins + kScanChunk - 1) / kScanChunk);
+  if (nb > kScanChunk) fail(FS2_ERR_VALUE, "too many paint bins (%lld)", (long long)nbins);
+  // cursor holds the per-block partial scan until k_scan_add rewrites it
+  launch("paint_scan", (double)nbins * 8.0, k_scan_blocks, dim3(nb), dim3(kScanThreads), 0,
+         (const unsigned*)s.counts, nbins, s.cursor, s.block_sums);
+  launch("paint_scan_sums", (double)nb * 8.0, k_scan_sums, dim3(1), dim3(kScanThreads), 0,
+         s.block_sums, nb, s.offsets, nbins);
+  launch("paint_scan_add", (double)nbins * 12.0, k_scan_add, dim3(blocks_for(nbins)),
+         dim3(kBlock), 0, (const unsigned*)s.cursor, (const unsigned*)s.block_sums, nbins,
+         s.offsets, s.cursor);
+  if (p.np > 0) {
+    launch("paint_scatter", (double)p.np * 28.0, k_scatter, dim3(blocks_for(p.np)), dim3(kBlock),
+           0, (const float*)p.x, p.stride, p.np, (const unsigned*)s.keys, s.cursor, s.sx,
+           (int64_t)s.np);
+  }
+  const int tcells = g.tx * g.ty * g.tz;
+  const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0;
+  if (deterministic) {
+    launch("paint_tile_det", bytes, k_tile_paint<unsigned long long>, dim3((unsigned)ntiles),
+           dim3(kBlock), tcells * sizeof(unsigned long long), (const float*)s.sx, (int64_t)s.np,
+           (const unsigned*)s.offsets, g, fac, grid);
+  } else {
+    launch("paint_tile", bytes, k_tile_paint<float>, dim3((unsigned)ntiles), dim3(kBlock),
+           tcells * sizeof(float), (const float*)s.sx, (int64_t)s.np, (const unsigned*)s.offsets,
+           g, fac, grid);
+  }
+}
+
+}  // namespace fs2

@@ -150,8 +150,8 @@ __global__ void k_grid_sum(const float* __restrict__ g, int n, int ncz, double* 
 }
 
 // Binned P(k) over the half-complex modes; LDS bins, FP64 sums.
-__global__ void k_power_spectrum(const float2* __restrict__ dk, int n, int nbins, double dk_bin,
-                                 double pnorm, double* __restrict__ out /* 3*nbins */) {
+__global__ void k_power_spectrum(const float2* __restrict__ dk, int n, int cpitch, int nbins,
+                                 double dk_bin, double pnorm, double* __restrict__ out /* 3*nbins */) {
   extern __shared__ double bins[];   // 3*nbins
   for (int b = threadIdx.x; b < 3 * nbins; b += blockDim.x) bins[b] = 0.0;
   __syncthreads();
@@ -170,7 +170,7 @@ __global__ void k_power_spectrum(const float2* __restrict__ dk, int n, int nbins
     const int b = (int)floor(kk / dk_bin);
     if (b >= nbins) continue;
     const double w = (iz == 0 || iz == n / 2) ? 1.0 : 2.0;
-    const float2 v = dk[idx];
+    const float2 v = dk[rxy * cpitch + iz];
     const double p = ((double)v.x * v.x + (double)v.y * v.y) * pnorm;
     atomicAdd(&bins[b], w);
     atomicAdd(&bins[nbins + b], w * kk);
@@ -184,7 +184,7 @@ __global__ void k_power_spectrum(const float2* __restrict__ dk, int n, int nbins
 }  // namespace
 
 void fill_grid(float* grid, int n, float value) {
-  const int64_t total = (int64_t)n * n * (2 * (n / 2 + 1));
+  const int64_t total = (int64_t)grid_floats(n);
   const int64_t n4 = total / 4;
   launch("fill_grid", (double)total * 4.0, k_fill, dim3(blocks_for(n4)), dim3(kBlock), 0,
          (float4*)grid, n4, value);
@@ -198,7 +198,7 @@ void wrap_positions(PosArgs p, float boxsize) {
 
 void paint_atomic(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap) {
   if (p.np == 0) return;
-  const int ncz = 2 * (n / 2 + 1);
+  const int ncz = grid_rpitch(n);
   // pm.cpp:77: Float dx_inv = nc/boxsize
   const float dx_inv = (float)n / boxsize;
   // algorithmic bytes: read x (12 B/particle); the mesh write is counted
@@ -211,7 +211,7 @@ void paint_atomic(PosArgs p, float* grid, int n, float boxsize, float fac, bool 
 void readout3(PosArgs p, const float* g0, const float* g1, const float* g2, float* f, int n,
               float boxsize) {
   if (p.np == 0) return;
-  const int ncz = 2 * (n / 2 + 1);
+  const int ncz = grid_rpitch(n);
   const float dx_inv = (float)n / boxsize;
   const double bytes = (double)p.np * 24.0 + 3.0 * 4.0 * (double)n * n * n;
   launch("readout3", bytes, k_readout3, dim3(blocks_for(p.np)), dim3(kBlock), 0,
@@ -224,7 +224,7 @@ double grid_sum(const float* grid, int n) {
   const int64_t rows = (int64_t)n * n;
   const unsigned nb = (unsigned)(rows < 4096 ? rows : 4096);
   launch("grid_sum", (double)n * n * n * 4.0, k_grid_sum, dim3(nb), dim3(kBlock), 0, grid, n,
-         2 * (n / 2 + 1), d);
+         grid_rpitch(n), d);
   double h = 0.0;
   FS2_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, stream()));
   FS2_HIP(hipStreamSynchronize(stream()));
@@ -243,7 +243,8 @@ void power_spectrum(const float* delta_k, int n, double boxsize, int nbins, doub
   const int64_t total = (int64_t)n * n * (n / 2 + 1);
   const unsigned nb = (unsigned)std::min<int64_t>(blocks_for(total), 2048);
   launch("power_spectrum", (double)total * 8.0, k_power_spectrum, dim3(nb), dim3(kBlock),
-         3 * nbins * sizeof(double), (const float2*)delta_k, n, nbins, dk_bin, pnorm, d);
+         3 * nbins * sizeof(double), (const float2*)delta_k, n, grid_cpitch(n), nbins, dk_bin,
+         pnorm, d);
   std::vector<double> h(3 * nbins);
   FS2_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * 3 * nbins, hipMemcpyDeviceToHost,
                          stream()));

@@ -4,6 +4,7 @@ from .fft import FFT
 
 PAINT_ATOMIC = 0
 PAINT_SORTED = 1
+PAINT_GLOBAL = 2
 
 
 def init(nc_pm, pm_factor, boxsize):
@@ -68,9 +69,12 @@ def set_packet_size(packet_size):
 
 
 def set_paint_variant(variant):
-    """PAINT_ATOMIC (default) or PAINT_SORTED (deterministic)."""
+    """'sorted' (default: tiled, deterministic int64 fixed-point tile sums),
+    'atomic' (tiled, FP32 LDS atomics) or 'global' (per-particle global
+    FP32 atomics on a mesh cleared to -1)."""
     if isinstance(variant, str):
-        variant = {'atomic': PAINT_ATOMIC, 'sorted': PAINT_SORTED}[variant]
+        variant = {'atomic': PAINT_ATOMIC, 'sorted': PAINT_SORTED,
+                   'global': PAINT_GLOBAL}[variant]
     c._pm_set_paint_variant(variant)
 
 

@@ -44,10 +44,20 @@ enum {
   FS2_ERR_INDEX = 7       /* IndexError     */
 };
 
-/* Paint variants for fs2_pm_set_paint_variant. */
+/* Paint variants for fs2_pm_set_paint_variant.  All replace the scatter
+ * loop of py/pm.cpp:57-139 and give the same density to FP32 rounding.
+ * The two tiled variants share the binning: particles are counting-sorted
+ * by the tile of their lower CIC cell (wavefront-aggregated atomics), and
+ * one workgroup per tile accumulates its cells in LDS and stores each cell
+ * once (no mesh clear, no global float atomics). */
 enum {
-  FS2_PAINT_ATOMIC = 0,        /* per-particle FP32 atomics (default)       */
-  FS2_PAINT_SORTED = 1         /* deterministic sort-by-cell segmented sum  */
+  FS2_PAINT_ATOMIC = 0,   /* tiled, LDS FP32 atomics; summation order varies
+                             run to run                                     */
+  FS2_PAINT_SORTED = 1,   /* default: tiled, per-tile segmented sums in int64
+                             fixed point (2^-32); bitwise reproducible and,
+                             on gfx950, faster than FP32 LDS atomics        */
+  FS2_PAINT_GLOBAL = 2    /* one thread per particle, 8 global FP32 atomics
+                             onto a mesh cleared to -1 (pm.cpp's loop as is) */
 };
 
 /* Particle fields for fs2_particles_get_field / set_field. */

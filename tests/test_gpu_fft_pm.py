@@ -165,3 +165,48 @@ def test_compute_force_requires_density(fs):
     # pm.cpp:314-317: status is force_done, not density_done
     with pytest.raises(RuntimeError):
         fs.pm.compute_force(p)
+
+
+def _clustered(npart, L, rng):
+    """Half uniform, half in a few tight clumps, plus particles on tile and
+    box faces (lower cells on the last layer of a tile spill into the next)."""
+    u = rng.uniform(0, L, (npart // 2, 3))
+    centres = rng.uniform(0, L, (5, 3))
+    c = centres[rng.integers(0, 5, npart - npart // 2)]
+    clump = np.mod(c + 0.01 * L * rng.standard_normal(c.shape), L)
+    x = np.concatenate([u, clump])
+    x[:64, 0] = L * (1 - 1e-7)          # rounds to cell n -> index 0
+    x[64:128, 1] = L * 15.5 / 16.0       # last layer of a 16-cell tile
+    return x
+
+
+@pytest.mark.parametrize("n", [8, 16, 48, 64, 96])
+def test_paint_variants_match_oracle(fs, n):
+    """Tiled (LDS atomics), deterministic (fixed point) and per-particle
+    global-atomic paints all reproduce the oracle; the deterministic one is
+    bitwise reproducible."""
+    _announce(f"paint variants n={n}")
+    rng = np.random.default_rng(7 * n)
+    L = 50.0
+    npart = 8 * n**3 // 8
+    x = _clustered(npart, L, rng)
+    fs.pm.init(n, 2.0, L)
+    p = fs.Particles(npart, L)
+    p.append(x)
+    ref = ocic.paint(ocic.wrap(x.astype(np.float32), L), n, L, 2.0, wdtype=np.float32)
+    # FP32 sums in any order over up to thousands of contributions per cell
+    # (clumps): per-cell bound 1e-5 relative.  Fixed-point sums are exact to
+    # 2^-32 per term, so the deterministic variant carries only the final
+    # FP32 rounding.
+    rho = np.abs(ref + 1.0)
+    out = {}
+    for variant in ("global", "atomic", "sorted"):
+        fs.pm.set_paint_variant(variant)
+        fs.pm.send_positions(p)
+        out[variant] = fs.pm.compute_density(p).asarray().astype(np.float64)
+        err = np.abs(out[variant] - ref)
+        assert np.all(err <= 1e-5 * rho + 1e-6), (variant, (err / (rho + 1e-6)).max())
+    assert np.all(np.abs(out["sorted"] - ref) <= 1.2e-7 * (rho + 1.0)), "sorted"
+    fs.pm.send_positions(p)
+    again = fs.pm.compute_density(p).asarray()
+    assert np.array_equal(again, out["sorted"].astype(np.float32))
