@@ -26,6 +26,7 @@ struct fs2_particles {
   float* planes = nullptr;   // x, v, dx1, dx2, force: each [3][np_alloc]
   uint64_t* id = nullptr;
   bool wrap_pending = false;
+  uint64_t x_version = 0;    // changes whenever positions change (see touch_x)
   float* field(int k) const { return planes + (size_t)k * 3 * np_alloc; }
   float* x() const { return field(0); }
   float* v() const { return field(1); }
@@ -48,6 +49,8 @@ struct fs2_pm {
   int variant = FS2_PAINT_SORTED;
   bool mesh_is_k = false;
   fs2::PaintScratch paint;     // tiled-paint scratch, grown on demand
+  uint64_t binned_version = 0; // x_version of the particles the scratch bins
+  uint64_t binned_np = 0;
 };
 
 namespace fs2 {
@@ -222,10 +225,16 @@ fs2_particles* check_p(fs2_particles* p) {
 
 PosArgs pos_of(fs2_particles* p) { return PosArgs{p->x(), (int64_t)p->np_alloc, (int64_t)p->np_local}; }
 
+// Positions changed: give them a version no other state ever had, so bins
+// computed from earlier positions are never reused.
+uint64_t g_version_counter = 0;
+void touch_x(fs2_particles* p) { p->x_version = ++g_version_counter; }
+
 void flush_wrap(fs2_particles* p) {
   if (p->wrap_pending) {
     wrap_positions(pos_of(p), (float)p->boxsize);
     p->wrap_pending = false;
+    touch_x(p);
   }
 }
 
@@ -283,7 +292,7 @@ void free_paint_scratch(PaintScratch& s) {
   dfree(s.offsets);
   dfree(s.cursor);
   dfree(s.block_sums);
-  dfree(s.sx);
+  dfree(s.rec);
   s = PaintScratch();
 }
 
@@ -297,7 +306,7 @@ void ensure_paint_scratch(PaintScratch& s, size_t np, size_t nbins) {
   s.offsets = (unsigned*)dmalloc((nbins + 1) * sizeof(unsigned));
   s.cursor = (unsigned*)dmalloc(nbins * sizeof(unsigned));
   s.block_sums = (unsigned*)dmalloc(4096 * sizeof(unsigned));
-  s.sx = (float*)dmalloc(3 * cap * sizeof(float));
+  s.rec = (float4*)dmalloc(cap * sizeof(float4));
   s.np = cap;
   s.nbins = nbins;
 }
@@ -464,6 +473,7 @@ int fs2_particles_append(fs2_particles* p, const double* x, uint64_t n, int64_t 
     sync();
     p->np_local += n;
     p->np_total = p->np_local;
+    touch_x(p);
   });
 }
 
@@ -473,6 +483,7 @@ int fs2_particles_clear(fs2_particles* p) {
     p->wrap_pending = false;
     p->np_local = 0;
     p->np_total = 0;
+    touch_x(p);
   });
 }
 
@@ -505,6 +516,7 @@ int fs2_particles_set_field(fs2_particles* p, int field, const float* in) {
       default: fail(FS2_ERR_VALUE, "unknown particle field %d", field);
     }
     upload_rows(d, p->np_alloc, p->np_local, in);
+    if (field == FS2_FIELD_X) touch_x(p);
   });
 }
 
@@ -553,6 +565,7 @@ int fs2_particles_periodic_wrapup(fs2_particles* p) {
     check_p(p);
     p->wrap_pending = false;
     wrap_positions(pos_of(p), (float)p->boxsize);
+    touch_x(p);
   });
 }
 
@@ -584,6 +597,7 @@ int fs2_particles_device_fields(fs2_particles* p, float** x, float** v, float** 
     if (force) *force = p->f();
     if (id) *id = p->id;
     if (stride) *stride = p->np_alloc;
+    touch_x(p);   // the caller may write the planes
   });
 }
 
@@ -602,6 +616,7 @@ int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src) {
     dst->a_v = src->a_v;
     dst->a_f = src->a_f;
     dst->wrap_pending = src->wrap_pending;
+    touch_x(dst);
   });
 }
 
@@ -733,7 +748,10 @@ int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density) {
       ensure_paint_scratch(pm->paint, p->np_local, paint_tiled_bins(n));
       paint_tiled(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending,
                   pm->variant == FS2_PAINT_SORTED, pm->paint);
+      if (p->wrap_pending) touch_x(p);   // the binning kernel wrapped x in place
     }
+    pm->binned_version = pm->variant == FS2_PAINT_GLOBAL ? 0 : p->x_version;
+    pm->binned_np = p->np_local;
     p->wrap_pending = false;
     pm->mesh_is_k = false;
     pm->status = PM_DENSITY_DONE;
@@ -762,11 +780,17 @@ int fs2_pm_compute_force(fs2_pm* pm, fs2_particles* p) {
     const int n = pm->nc;
     // pm.cpp:472: f1 = -1/n^3 / (2 pi / boxsize), boxsize held as a Float
     const float f1 = (float)(-1.0 / std::pow((double)n, 3.0) / (2.0 * M_PI / (double)pm->boxsize_f));
-    fft_r2c_3d(pm->mesh, n);
-    pm->mesh_is_k = true;
-    for (int a = 0; a < 3; ++a) fft_green_c2r_3d(pm->mesh, pm->force[a], n, a, f1);
+    fft_force_meshes(pm->mesh, pm->force, n, f1);
+    pm->mesh_is_k = false;
+    pm->status = PM_FORCE_DONE;   // the density mesh no longer holds delta
     flush_wrap(p);
-    readout3(pos_of(p), pm->force[0], pm->force[1], pm->force[2], p->f(), n, pm->boxsize_f);
+    if (pm->binned_version != 0 && pm->binned_version == p->x_version &&
+        pm->binned_np == p->np_local) {
+      readout_tiled(pm->force[0], pm->force[1], pm->force[2], p->f(), (int64_t)p->np_alloc,
+                    (int64_t)p->np_local, n, pm->boxsize_f, pm->paint);
+    } else {
+      readout3(pos_of(p), pm->force[0], pm->force[1], pm->force[2], p->f(), n, pm->boxsize_f);
+    }
     p->a_f = p->a_x;
     pm->status = PM_FORCE_DONE;
   });
@@ -828,6 +852,7 @@ int fs2_cola_drift(fs2_particles* p, double a_pos) {
     // cola.cpp:106-109: da1, da2 stored as Float; dt stays double
     cola_drift(p->x(), p->v(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local, dt,
                (float)da1, (float)da2);
+    touch_x(p);
     p->a_x = a_pos;
   });
 }
@@ -961,6 +986,7 @@ int fs2_lpt(int nc, double boxsize, double a, uint64_t seed, const fs2_power* ps
       p->np_total = np;
       p->a_x = a;
       p->a_v = a;
+      touch_x(p);
     } catch (...) {
       cleanup();
       fs2_particles_free(p);

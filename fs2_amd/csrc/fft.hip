@@ -198,6 +198,13 @@ __device__ __forceinline__ void fft_lds(float2* s, int t, const float2* __restri
   run_plan<N, Plan<N>::T, L>(s, t, tw, typename Plan<N>::R{});
 }
 
+// Copies the N-entry twiddle table into LDS (read there by every stage; a
+// barrier follows before first use).
+template <int N>
+__device__ __forceinline__ void stage_twiddles(float2* stw, const float2* __restrict__ tw) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) stw[i] = tw[i];
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that
 // consecutive logical tiles run on the same XCD and share its L2.
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
@@ -236,30 +243,99 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
   constexpr int T = Plan<N>::T;
   using L = RowsLayout<P>;
   __shared__ float2 s[L::rows(N) * P];
+  __shared__ float2 stw[N];
+  stage_twiddles<N>(stw, tw);
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
   const int bid = xcd_remap(blockIdx.x, nblocks);
   const int o = bid / nchunks;
   const int c = (bid - o * nchunks) * P + p;
   const bool valid = c < ncols;
-  const int64_t base = (int64_t)o * outer_stride + c;
+  // 32-bit element offsets: grids up to 1024^3 stay below 2^31 float2
+  const int base = o * (int)outer_stride + c;
+  const int es = (int)elem_stride;
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
     const int j = t + i * T;
     float2 v = make_float2(0.f, 0.f);
-    if (valid) v = src[base + (int64_t)j * elem_stride];
+    if (valid) v = src[base + j * es];
     if constexpr (GREEN) v = green_mul(v, j, o, c, g);
     if constexpr (INV) v.y = -v.y;
     s[L::at(j) + p] = v;
   }
   __syncthreads();
-  fft_lds<N, L>(s + p, t, tw);
+  fft_lds<N, L>(s + p, t, stw);
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
     const int j = t + i * T;
     float2 v = s[L::at(j) + p];
     if constexpr (INV) v.y = -v.y;
-    if (valid) dst[base + (int64_t)j * elem_stride] = v;
+    if (valid) dst[base + j * es] = v;
+  }
+}
+
+// Forward x pass fused with the three Green x gradient inverse x passes
+// (py/pm.cpp:431-504): each x pencil of the y-transformed mesh is read once,
+// transformed forward in LDS, kept in registers as delta_k, and for each
+// axis a multiplied by i f2 k_a / |k|^2 and transformed back into the
+// corresponding force mesh.  Saves the delta_k write and its three reads.
+template <int N, int P>
+__global__ void __launch_bounds__(Plan<N>::T * P)
+k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int ncols,
+               int64_t elem_stride, int64_t outer_stride, int nchunks, int nblocks,
+               const float2* __restrict__ tw, float f1) {
+  constexpr int T = Plan<N>::T;
+  using L = RowsLayout<P>;
+  __shared__ float2 s[L::rows(N) * P];
+  __shared__ float2 stw[N];
+  stage_twiddles<N>(stw, tw);
+  const int tid = threadIdx.x;
+  const int p = tid % P, t = tid / P;
+  const int bid = xcd_remap(blockIdx.x, nblocks);
+  const int o = bid / nchunks;
+  const int c = (bid - o * nchunks) * P + p;
+  const bool valid = c < ncols;
+  const int base = o * (int)outer_stride + c;
+  const int es = (int)elem_stride;
+#pragma unroll
+  for (int i = 0; i < N / T; ++i) {
+    const int j = t + i * T;
+    float2 v = make_float2(0.f, 0.f);
+    if (valid) v = src[base + j * es];
+    s[L::at(j) + p] = v;
+  }
+  __syncthreads();
+  fft_lds<N, L>(s + p, t, stw);
+  float2 dk[N / T];
+#pragma unroll
+  for (int i = 0; i < N / T; ++i) dk[i] = s[L::at(t + i * T) + p];
+  float2* dsts[3] = {dst0, dst1, dst2};
+#pragma unroll 1
+  for (int a = 0; a < 3; ++a) {
+    const GreenArg g{a, N, f1};
+    // Opaque copies of the thread coordinates: without them the compiler
+    // hoists every per-stage LDS address out of this loop and keeps ~100
+    // of them live, halving occupancy.
+    int tl = t, pl = p;
+    asm volatile("" : "+v"(tl), "+v"(pl));
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N / T; ++i) {
+      const int j = tl + i * T;
+      float2 v = green_mul(dk[i], j, o, c, g);
+      v.y = -v.y;
+      s[L::at(j) + pl] = v;
+    }
+    __syncthreads();
+    fft_lds<N, L>(s + pl, tl, stw);
+    float2* dst = dsts[a];
+#pragma unroll
+    for (int i = 0; i < N / T; ++i) {
+      const int j = tl + i * T;
+      float2 v = s[L::at(j) + pl];
+      v.y = -v.y;
+      if (valid) dst[base + j * es] = v;
+    }
   }
 }
 
@@ -435,6 +511,28 @@ struct Impl {
     z_c2r(grid);
   }
 
+  // density -> three real force meshes (pm.cpp:431-508)
+  static void force_meshes(float* mesh, float* const out[3], float f1) {
+    const int64_t nh = grid_cpitch(n);
+    float2* c = (float2*)mesh;
+    GreenArg g{0, n, 0.f};
+    z_r2c(mesh);
+    pencil<false, false>("fft_y_fwd", c, c, nh, (int64_t)n * nh, g);
+    {
+      const int ncols = n / 2 + 1;
+      const int nchunks = (ncols + P - 1) / P;
+      const int nblocks = n * nchunks;
+      launch("fft_x_fwd_green3", 4.0 * grid_bytes(), k_fft_x_green3<N, P>, dim3(nblocks),
+             dim3(Plan<N>::T * P), 0, (const float2*)c, (float2*)out[0], (float2*)out[1],
+             (float2*)out[2], ncols, (int64_t)n * nh, nh, nchunks, nblocks, twiddles(N), f1);
+    }
+    for (int a = 0; a < 3; ++a) {
+      float2* o = (float2*)out[a];
+      pencil<true, false>("fft_y_inv", o, o, nh, (int64_t)n * nh, g);
+      z_c2r(out[a]);
+    }
+  }
+
   static void green_c2r(const float* dk, float* dst, int axis, float f1) {
     const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)dst;
@@ -465,6 +563,9 @@ void dispatch(int n, F&& f) {
 
 }  // namespace
 
+static_assert(1024LL * 1024 * (((1024 / 2 + 1) + 15) & ~15) < (1LL << 31),
+              "32-bit float2 offsets in the pencil kernels");
+
 bool fft_supported(int n) {
   switch (n) {
     case 4: case 8: case 16: case 32: case 48: case 64: case 96: case 128: case 256:
@@ -481,6 +582,10 @@ void fft_r2c_3d(float* grid, int n) {
 
 void fft_c2r_3d(float* grid, int n) {
   dispatch(n, [&](auto impl) { decltype(impl)::c2r(grid); });
+}
+
+void fft_force_meshes(float* mesh, float* const out[3], int n, float f1) {
+  dispatch(n, [&](auto impl) { decltype(impl)::force_meshes(mesh, out, f1); });
 }
 
 void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f1) {

@@ -78,6 +78,11 @@ void fft_c2r_3d(float* grid, int n);
 //   dst = IFFT_x( i f2(k) delta_k ),  f2 = f1 k_axis / |k|^2, Nyquist -> +n/2
 // then the y pass and the z c2r pass in place on dst.
 void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f1);
+// The whole PM mesh transform (pm.cpp:431-508): r2c of the density in place
+// (z and y passes), then one fused x pass that writes the three Green x
+// gradient spectra, then y and z c2r passes of each force mesh.  The density
+// mesh is left holding its y-transformed intermediate.
+void fft_force_meshes(float* mesh, float* const out[3], int n, float f1);
 
 // ---- PM kernels (pm_kernels.hip) -----------------------------------------------
 struct PosArgs {
@@ -100,12 +105,16 @@ struct PaintScratch {
   unsigned* offsets = nullptr;   // [nbins + 1]
   unsigned* cursor = nullptr;    // [nbins]
   unsigned* block_sums = nullptr;   // [4096] scan block totals
-  float* sx = nullptr;           // [3][np] positions in bin order
+  float4* rec = nullptr;         // [np] (x, y, z, particle index) in bin order
   size_t np = 0, nbins = 0;
 };
 size_t paint_tiled_bins(int n);
 void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
                  bool deterministic, const PaintScratch& s);
+// Readout of the three force meshes at the particles binned by the last
+// paint_tiled call (positions unchanged since); forces land at f[i + a*fstride].
+void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, int64_t fstride,
+                   int64_t np, int n, float boxsize, const PaintScratch& s);
 void power_spectrum(const float* delta_k, int n, double boxsize, int nbins,
                     double dk_bin, double* k, double* P, double* nmodes);
 

@@ -211,16 +211,18 @@ __global__ void k_scan_add(const unsigned* partial, const unsigned* __restrict__
   cursor[i] = v;
 }
 
+// One 16-byte record per particle in bin order: position and the particle's
+// index (bits in .w), which the tiled readout uses to write forces back in
+// id order.
 __global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t np,
                           const unsigned* __restrict__ keys, unsigned* __restrict__ cursor,
-                          float* __restrict__ sx, int64_t sstride) {
+                          float4* __restrict__ rec) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < np;
   const unsigned key = valid ? keys[i] : kNoKey;
   const unsigned slot = wave_aggregated_inc(cursor, key, valid);
   if (valid) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) sx[slot + a * sstride] = x[i + a * stride];
+    rec[slot] = make_float4(x[i], x[i + stride], x[i + 2 * stride], __uint_as_float((unsigned)i));
   }
 }
 
@@ -228,7 +230,7 @@ __global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t n
 // unsigned long long (int64 fixed point, deterministic).
 template <typename ACC>
 __global__ void __launch_bounds__(kBlock)
-k_tile_paint(const float* __restrict__ sx, int64_t sstride, const unsigned* __restrict__ offsets,
+k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
              TileGeom g, float fac, float* __restrict__ grid) {
   extern __shared__ unsigned char smem[];
   ACC* acc = (ACC*)smem;
@@ -296,9 +298,10 @@ k_tile_paint(const float* __restrict__ sx, int64_t sstride, const unsigned* __re
   for (unsigned v = threadIdx.x; v < total; v += blockDim.x) {
     while (v >= rpre[r + 1]) ++r;
     const unsigned j = rb[r] + (v - rpre[r]);
-    const Cic1 cx = cic1(sx[j], g.dx_inv, g.n);
-    const Cic1 cy = cic1(sx[j + sstride], g.dx_inv, g.n);
-    const Cic1 cz = cic1(sx[j + 2 * sstride], g.dx_inv, g.n);
+    const float4 r4 = rec[j];
+    const Cic1 cx = cic1(r4.x, g.dx_inv, g.n);
+    const Cic1 cy = cic1(r4.y, g.dx_inv, g.n);
+    const Cic1 cz = cic1(r4.z, g.dx_inv, g.n);
     const int lx[2] = {cx.i0 - x0, cx.i1 - x0};
     const int ly[2] = {cy.i0 - y0, cy.i1 - y0};
     const int lz[2] = {cz.i0 - z0, cz.i1 - z0};
@@ -338,6 +341,62 @@ k_tile_paint(const float* __restrict__ sx, int64_t sstride, const unsigned* __re
       v = (float)((double)(long long)acc[c] * (1.0 / kFix));
     }
     grid[((int64_t)(x0 + lx) * g.n + (y0 + ly)) * g.ncz + (z0 + lz)] = v - 1.0f;
+  }
+}
+
+// Tiled CIC readout (force_at_particle_locations, py/pm.cpp:142-220, all
+// three axes): one workgroup per tile stages the three force meshes over the
+// tile plus its +1 halo in LDS, then interpolates at the tile's own
+// particles (every particle is in exactly one tile's bins) and writes the
+// forces back to the particles' id-order slots.  Same FP32 expression order
+// as the gather kernel.
+__global__ void __launch_bounds__(512)
+k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offsets, TileGeom g,
+               const float* __restrict__ m0, const float* __restrict__ m1,
+               const float* __restrict__ m2, float* __restrict__ f, int64_t fstride) {
+  extern __shared__ float hs[];
+  const int hx = g.tx + 1, hy = g.ty + 1, hz = g.tz + 1;
+  const int hcells = hx * hy * hz;
+  const int tile = blockIdx.x;
+  const int Tz = tile % g.ntz;
+  const int Ty = (tile / g.ntz) % g.nty;
+  const int Tx = tile / (g.ntz * g.nty);
+  const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
+  const float* ms[3] = {m0, m1, m2};
+  for (int c = threadIdx.x; c < hcells; c += blockDim.x) {
+    const int lz = c % hz;
+    const int r = c / hz;
+    const int ly = r % hy, lx = r / hy;
+    int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    if (gx >= g.n) gx -= g.n;
+    if (gy >= g.n) gy -= g.n;
+    if (gz >= g.n) gz -= g.n;
+    const int64_t gi = ((int64_t)gx * g.n + gy) * g.ncz + gz;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) hs[a * hcells + c] = ms[a][gi];
+  }
+  __syncthreads();
+  const unsigned b = offsets[(unsigned)tile * 8u], e = offsets[(unsigned)tile * 8u + 8u];
+  for (unsigned j = b + threadIdx.x; j < e; j += blockDim.x) {
+    const float4 r4 = rec[j];
+    const Cic1 cx = cic1(r4.x, g.dx_inv, g.n);
+    const Cic1 cy = cic1(r4.y, g.dx_inv, g.n);
+    const Cic1 cz = cic1(r4.z, g.dx_inv, g.n);
+    const int lx = cx.i0 - x0, ly = cy.i0 - y0, lz = cz.i0 - z0;
+    const int c000 = (lx * hy + ly) * hz + lz;
+    const int c010 = c000 + hz, c100 = c000 + hy * hz, c110 = c100 + hz;
+    const unsigned i = __float_as_uint(r4.w);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float* h = hs + a * hcells;
+      // pm.cpp:203-218: f = 0; f += (ix0 terms); f += (ix1 terms)
+      float s = 0.f;
+      s += h[c000] * cx.w0 * cy.w0 * cz.w0 + h[c000 + 1] * cx.w0 * cy.w0 * cz.w1 +
+           h[c010] * cx.w0 * cy.w1 * cz.w0 + h[c010 + 1] * cx.w0 * cy.w1 * cz.w1;
+      s += h[c100] * cx.w1 * cy.w0 * cz.w0 + h[c100 + 1] * cx.w1 * cy.w0 * cz.w1 +
+           h[c110] * cx.w1 * cy.w1 * cz.w0 + h[c110 + 1] * cx.w1 * cy.w1 * cz.w1;
+      f[i + a * fstride] = s;
+    }
   }
 }
 
@@ -394,20 +453,29 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
          s.offsets, s.cursor);
   if (p.np > 0) {
     launch("paint_scatter", (double)p.np * 28.0, k_scatter, dim3(blocks_for(p.np)), dim3(kBlock),
-           0, (const float*)p.x, p.stride, p.np, (const unsigned*)s.keys, s.cursor, s.sx,
-           (int64_t)s.np);
+           0, (const float*)p.x, p.stride, p.np, (const unsigned*)s.keys, s.cursor, s.rec);
   }
   const int tcells = g.tx * g.ty * g.tz;
   const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0;
   if (deterministic) {
     launch("paint_tile_det", bytes, k_tile_paint<unsigned long long>, dim3((unsigned)ntiles),
-           dim3(kBlock), tcells * sizeof(unsigned long long), (const float*)s.sx, (int64_t)s.np,
+           dim3(kBlock), tcells * sizeof(unsigned long long), (const float4*)s.rec,
            (const unsigned*)s.offsets, g, fac, grid);
   } else {
     launch("paint_tile", bytes, k_tile_paint<float>, dim3((unsigned)ntiles), dim3(kBlock),
-           tcells * sizeof(float), (const float*)s.sx, (int64_t)s.np, (const unsigned*)s.offsets,
+           tcells * sizeof(float), (const float4*)s.rec, (const unsigned*)s.offsets,
            g, fac, grid);
   }
+}
+
+void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, int64_t fstride,
+                   int64_t np, int n, float boxsize, const PaintScratch& s) {
+  const TileGeom g = make_geom(n, boxsize);
+  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
+  const size_t lds = (size_t)3 * (g.tx + 1) * (g.ty + 1) * (g.tz + 1) * sizeof(float);
+  const double bytes = (double)np * 24.0 + 3.0 * 4.0 * (double)n * n * n;
+  launch("readout_tile", bytes, k_tile_readout, dim3((unsigned)ntiles), dim3(512), lds,
+         (const float4*)s.rec, (const unsigned*)s.offsets, g, g0, g1, g2, f, fstride);
 }
 
 }  // namespace fs2

@@ -210,3 +210,31 @@ def test_paint_variants_match_oracle(fs, n):
     fs.pm.send_positions(p)
     again = fs.pm.compute_density(p).asarray()
     assert np.array_equal(again, out["sorted"].astype(np.float32))
+
+
+@pytest.mark.parametrize("n_p,nc", [(16, 32), (24, 48)])
+def test_tiled_readout_matches_gather_readout(fs, n_p, nc):
+    """compute_force reads the force meshes through the tile kernel when the
+    paint's bins are current and through the per-particle gather otherwise;
+    with the deterministic paint both see the same meshes and must agree
+    bitwise."""
+    _announce(f"tiled vs gather readout n_p={n_p} nc={nc}")
+    L = 40.0
+    x = np.mod(_lattice_positions(n_p, L, 0.8, seed=3 * nc), L)
+    fs.pm.init(nc, nc / n_p, L)
+    fs.pm.set_paint_variant("sorted")
+    p = fs.Particles(n_p**3, L)
+    p.append(x)
+    p.set_times(0.5, 0.5, 0.0)
+    fs.pm.force(p)                 # bins current -> tile readout
+    f_tile = p.force
+    p.set_times(0.5, 0.5, -1.0)
+    fs.pm.send_positions(p)
+    fs.pm.compute_density(p)
+    p.set_field("x", p.x)          # same values, new version -> gather readout
+    fs.pm.compute_force(p)
+    f_gather = p.force
+    assert np.array_equal(f_tile, f_gather)
+    ref = opm.force(p.x.astype(np.float64), nc, L, nc / n_p)
+    rel = np.sqrt(np.mean((f_tile - ref) ** 2) / np.mean(ref**2))
+    assert rel < 1e-5, rel
