@@ -10,3 +10,14 @@ import os
 
 LIBDIR = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(LIBDIR, 'libfs2hip.so')
+
+# One HIP runtime per process.  PyTorch-ROCm ships its own libamdhip64
+# (soname libamdhip64.so.7) and its libraries ask for it by the file name
+# libamdhip64.so, so if libfs2hip.so were loaded first (binding
+# /opt/rocm/lib/libamdhip64.so.7) and torch second, the process would hold
+# two HIP runtimes and the second to start could not open the device.
+# Importing torch first makes libfs2hip.so bind to torch's runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass

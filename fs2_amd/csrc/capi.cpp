@@ -54,7 +54,6 @@ struct fs2_pm {
 };
 
 namespace fs2 {
-double power_P(const fs2_power* ps, double k);
 fs2_power* power_load(const char* filename);
 fs2_power* power_eh98(double om, double ob, double h, double ns, double sigma8);
 double power_sigma(const fs2_power* ps, double R);
@@ -78,22 +77,6 @@ void fail(int code, const char* fmt, ...) {
   throw Error{code, std::string(buf)};
 }
 
-template <typename F>
-int guard(F&& f) {
-  try {
-    f();
-    return FS2_OK;
-  } catch (const Error& e) {
-    set_error(e.code, e.msg);
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    set_error(FS2_ERR_MEMORY, "host allocation failed");
-    return FS2_ERR_MEMORY;
-  } catch (const std::exception& e) {
-    set_error(FS2_ERR_RUNTIME, e.what());
-    return FS2_ERR_RUNTIME;
-  }
-}
 
 // ---- runtime ----------------------------------------------------------------------------
 namespace {
@@ -286,6 +269,8 @@ void grid_set_real(fs2_grid* g, const float* in) {
 
 float lpt_offset = 0.5f;   // py/lpt.cpp:29
 
+}  // namespace
+
 void free_paint_scratch(PaintScratch& s) {
   dfree(s.keys);
   dfree(s.counts);
@@ -311,7 +296,45 @@ void ensure_paint_scratch(PaintScratch& s, size_t np, size_t nbins) {
   s.nbins = nbins;
 }
 
-}  // namespace
+float lpt_offset_value() { return lpt_offset; }
+
+std::vector<double> lpt_amp_table(int n, double boxsize, const fs2_power* ps) {
+  const int h = n / 2;
+  const int nk2 = 3 * h * h + 1;
+  std::vector<double> amp(nk2, 0.0);
+  const double kf = 2.0 * M_PI / boxsize;
+  const double V = boxsize * boxsize * boxsize;
+  const double norm = 1.0 / std::pow((double)n, 1.5);
+  std::vector<char> used(nk2, 0);
+  for (int ix = -h + 1; ix < h; ++ix)
+    for (int iy = -h + 1; iy < h; ++iy)
+      for (int iz = 0; iz < h; ++iz) used[ix * ix + iy * iy + iz * iz] = 1;
+  for (int q = 1; q < nk2; ++q)
+    if (used[q]) amp[q] = std::sqrt(power_P(ps, kf * std::sqrt((double)q)) / V) * norm;
+  return amp;
+}
+
+void lpt_psi_fields(float* S, float* const G[6], int n, double boxsize, const double* d_amp) {
+  fft_r2c_3d(S, n);
+  lpt_psi_ij(S, G, n, boxsize, d_amp);
+  for (int i = 0; i < 6; ++i) fft_c2r_3d(G[i], n);
+}
+
+void lpt_displacements(float* S, float* const G[6], int n, double boxsize, const double* d_amp) {
+  lpt_psi_fields(S, G, n, boxsize, d_amp);
+  const float* cg[6] = {G[0], G[1], G[2], G[3], G[4], G[5]};
+  lpt_div2(G[0], cg, n);
+  fft_r2c_3d(G[0], n);
+  float* o5[5] = {G[1], G[2], G[3], G[4], G[5]};
+  lpt_psi12(S, G[0], o5, n, boxsize, d_amp);
+  fft_c2r_3d(G[1], n);
+  fft_c2r_3d(G[2], n);
+  fft_c2r_3d(S, n);
+  fft_c2r_3d(G[3], n);
+  fft_c2r_3d(G[4], n);
+  fft_c2r_3d(G[5], n);
+}
+
 }  // namespace fs2
 
 using namespace fs2;
@@ -917,19 +940,8 @@ int fs2_lpt(int nc, double boxsize, double a, uint64_t seed, const fs2_power* ps
     if (boxsize <= 0) fail(FS2_ERR_VALUE, "boxsize must be positive");
     (void)host::omega_m();   // cosmology must be initialised
 
-    // amplitude table indexed by integer |k|^2 (non-Nyquist modes only)
-    const int h = nc / 2;
-    const int nk2 = 3 * h * h + 1;
-    std::vector<double> amp(nk2, 0.0);
-    const double kf = 2.0 * M_PI / boxsize;
-    const double V = boxsize * boxsize * boxsize;
-    const double norm = 1.0 / std::pow((double)nc, 1.5);
-    std::vector<char> used(nk2, 0);
-    for (int ix = -h + 1; ix < h; ++ix)
-      for (int iy = -h + 1; iy < h; ++iy)
-        for (int iz = 0; iz < h; ++iz) used[ix * ix + iy * iy + iz * iz] = 1;
-    for (int q = 1; q < nk2; ++q)
-      if (used[q]) amp[q] = std::sqrt(power_P(ps, kf * std::sqrt((double)q)) / V) * norm;
+    const std::vector<double> amp = lpt_amp_table(nc, boxsize, ps);
+    const int nk2 = (int)amp.size();
 
     const size_t gbytes = grid_floats(nc) * sizeof(float);
     std::vector<float*> bufs;
@@ -947,20 +959,7 @@ int fs2_lpt(int nc, double boxsize, double a, uint64_t seed, const fs2_power* ps
       float* S = bufs[0];
       float* G[6] = {bufs[1], bufs[2], bufs[3], bufs[4], bufs[5], bufs[6]};
       white_noise(S, nc, seed);
-      fft_r2c_3d(S, nc);
-      lpt_psi_ij(S, G, nc, boxsize, d_amp);
-      for (int i = 0; i < 6; ++i) fft_c2r_3d(G[i], nc);
-      const float* cg[6] = {G[0], G[1], G[2], G[3], G[4], G[5]};
-      lpt_div2(G[0], cg, nc);
-      fft_r2c_3d(G[0], nc);
-      float* o5[5] = {G[1], G[2], G[3], G[4], G[5]};
-      lpt_psi12(S, G[0], o5, nc, boxsize, d_amp);
-      fft_c2r_3d(G[1], nc);
-      fft_c2r_3d(G[2], nc);
-      fft_c2r_3d(S, nc);
-      fft_c2r_3d(G[3], nc);
-      fft_c2r_3d(G[4], nc);
-      fft_c2r_3d(G[5], nc);
+      lpt_displacements(S, G, nc, boxsize, d_amp);
 
       // lpt.cpp:156-175: Float D1; D2, Dv, D2v evaluated from the Float D1
       const float D1 = (float)host::D_growth(a);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -34,6 +36,24 @@ void set_error(int code, const std::string& msg);
       ::fs2::fail(FS2_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,          \
                   hipGetErrorString(e_), __FILE__, __LINE__);                \
   } while (0)
+
+// Runs f, turning exceptions into a status code and the fs2_last_error text.
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return FS2_OK;
+  } catch (const Error& e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(FS2_ERR_MEMORY, "host allocation failed");
+    return FS2_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    set_error(FS2_ERR_RUNTIME, e.what());
+    return FS2_ERR_RUNTIME;
+  }
+}
 
 // ---- runtime --------------------------------------------------------------
 void device_init();                 // throws Error(FS2_ERR_DEVICE) without a GPU
@@ -109,6 +129,8 @@ struct PaintScratch {
   size_t np = 0, nbins = 0;
 };
 size_t paint_tiled_bins(int n);
+void ensure_paint_scratch(PaintScratch& s, size_t np, size_t nbins);   // grows on demand
+void free_paint_scratch(PaintScratch& s);
 void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
                  bool deterministic, const PaintScratch& s);
 // Readout of the three force meshes at the particles binned by the last
@@ -139,9 +161,42 @@ struct LatticeArgs {
   uint64_t* id;
   int64_t stride;
 };
+// out.id may be null (no ids written).
 void lpt_lattice(LatticeArgs out, const float* const psi1[3], const float* const psi2[3],
                  int n, float dx, float offset, float D1, float D2, float Dv, float D2v,
                  double nmesh3_inv);
+
+// Host orchestration of the 2LPT displacement fields (capi.cpp).
+// lpt_amp_table: sqrt(P(k)/V)/n^1.5 indexed by integer |k|^2 for the
+// non-Nyquist modes (py/lpt.cpp:328-331).
+std::vector<double> lpt_amp_table(int n, double boxsize, const fs2_power* ps);
+// S holds the real white noise on entry.  On return psi1 = {G[1], G[2], S}
+// and psi2 = {G[3], G[4], G[5]} hold the real first- and (n^3-scaled)
+// second-order displacements; G[0] holds the spectrum of div Psi2.
+void lpt_displacements(float* S, float* const G[6], int n, double boxsize, const double* d_amp);
+// As lpt_displacements up to the six real Psi_ij fields (G[0..5]); S is
+// left holding delta_k.
+void lpt_psi_fields(float* S, float* const G[6], int n, double boxsize, const double* d_amp);
+float lpt_offset_value();   // lattice offset in cells (fs2_lpt_set_offset)
+double power_P(const fs2_power* ps, double k);
+
+// ---- adjoint kernels (adjoint_kernels.hip) -------------------------------------------
+// Vector-Jacobian products of the differentiable COLA run.  Particle arrays
+// are planes [3][stride]; meshes are padded grids of size n.
+void adj_paint_values3(const float* x, const float* val, int64_t stride, int64_t np,
+                       float* const out[3], int n, float boxsize);
+void adj_grad_readout(const float* x, int64_t stride, int64_t np, const float* rhobar, float fac,
+                      const float* const F[3], const float* fbar, float* xbar, int n,
+                      float boxsize);
+void adj_green_sum(float* const a[3], int n, float f1);
+void adj_cola_step(const float* xbar, float* vbar, float* dx1bar, float* dx2bar, float* fbar,
+                   int64_t n3, float dt, float da1, float da2, float c, float q1, float q2);
+void adj_axpy(float* y, const float* x, float a, int64_t n);
+void plane_to_grid(const float* plane, float* grid, int n);
+void adj_lpt_div2(const float* divbar, float* const psi[6], int n);
+void adj_lpt_divbar_k(float* const g[3], int n, double boxsize);
+void adj_lpt_noisebar_k(float* const h[3], float* const q[6], int n, double boxsize,
+                        const double* amp);
 
 // ---- host numerics (host_numerics.cpp) ------------------------------------------------
 namespace host {

@@ -188,7 +188,7 @@ __global__ void k_lattice(LatticeArgs o, const float* __restrict__ s0, const flo
     o.dx1[i + k * o.stride] = dis;
     o.dx2[i + k * o.stride] = dis2;
   }
-  o.id[i] = (uint64_t)i + 1;
+  if (o.id) o.id[i] = (uint64_t)i + 1;
 }
 
 }  // namespace

@@ -70,6 +70,7 @@ typedef struct fs2_particles fs2_particles;
 typedef struct fs2_pm fs2_pm;
 typedef struct fs2_grid fs2_grid;
 typedef struct fs2_power fs2_power;
+typedef struct fs2_diff fs2_diff;
 
 /* ---- library ---------------------------------------------------------- */
 const char* fs2_last_error(void);
@@ -236,6 +237,28 @@ int fs2_lpt_set_zeldovich_force(fs2_particles* p, double a);
 /* The white-noise field of fs2_lpt for (nc, seed), (nc)^3 FP32 on the host
  * (no reference counterpart; exposes the field for checks). */
 int fs2_lpt_white_noise(int nc, uint64_t seed, float* out);
+
+/* ---- differentiable COLA run ---------------------------------------------- */
+/* No reference counterpart: fs2 has no adjoint.  The forward is the
+ * reference's 'cola' pipeline (py/lpt.cpp 2LPT with v0 = 0, then nstep
+ * kick/drift steps of py/cola.cpp:48-127 on the schedule of
+ * script/cola.py:37-44, PM forces of py/pm.cpp, final CIC density of
+ * py/pm.cpp:57-139); the backward is its vector-Jacobian product, which
+ * the reference lacks.  All buffers are DEVICE pointers on the library
+ * stream (fs2_get_stream); both calls synchronize that stream on return.
+ *   noise:     nc_lpt^3 FP32, [ix][iy][iz] contiguous (the white-noise field)
+ *   delta:     nc_pm^3 FP32, the final density contrast
+ * The handle keeps the positions of every step of the last forward
+ * (12 x nc_lpt^3 x (nstep + 1) bytes); fs2_diff_backward differentiates that
+ * forward and fails with FS2_ERR_RUNTIME if none ran.  Requires
+ * fs2_cosmology_init. */
+int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1, int nstep,
+                    const fs2_power* ps, fs2_diff** out);
+void fs2_diff_destroy(fs2_diff* d);
+int fs2_diff_params(const fs2_diff* d, int* nc_lpt, int* nc_pm, int* nstep, uint64_t* np);
+int fs2_diff_forward(fs2_diff* d, const float* noise, float* delta);
+/* delta_bar: d loss / d delta (nc_pm^3); noise_bar: d loss / d noise (nc_lpt^3). */
+int fs2_diff_backward(fs2_diff* d, const float* delta_bar, float* noise_bar);
 
 /* ---- kernel timing --------------------------------------------------------- */
 /* No reference counterpart (the reference has no timers).  When enabled,

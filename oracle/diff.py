@@ -160,4 +160,4 @@ def loss_and_grad(noise_np, target_np, boxsize, nc, a0, a1, nstep, Pfun, cosmo):
     delta = cola_density(noise, boxsize, nc, a0, a1, nstep, Pfun, cosmo)
     loss = torch.mean((delta - target) ** 2)
     loss.backward()
-    return float(loss), noise.grad.numpy().copy(), delta.detach().numpy()
+    return float(loss.detach()), noise.grad.numpy().copy(), delta.detach().numpy()
