@@ -37,6 +37,61 @@ def cola_step(fs, p, a_vel, a_pos):
     fs.cola.drift(p, a_pos)
 
 
+def adjoint_leg(fs, ps, reps=5):
+    """Differentiable COLA: 128^3 particles, 256^3 mesh, box 256 Mpc/h, 10 steps
+    a = 0.1 -> 1; loss = mean((delta - target)^2) against the final density of
+    a second seed.  Times the checkpointing forward and the adjoint sweep
+    (gradient with respect to the 128^3 white noise), both device-resident,
+    for the two checkpoint policies: 'store' (every step's force meshes and
+    bins kept; the default whenever it fits, 2.6 GB here) and 'recompute'
+    (positions only)."""
+    import torch
+    from fs2_amd.autograd import ColaModel
+    c = fs._fs
+    n, nm, L, nstep = 128, 256, 256.0, 10
+    res = {"workload": f"differentiable COLA {n}^3 particles, {nm}^3 mesh, box {L:g} Mpc/h, "
+                       f"{nstep} steps a=0.1->1, d MSE(delta, target) / d noise"}
+    for policy in ("store", "recompute"):
+        model = ColaModel(n, nm, L, 0.1, 1.0, nstep, ps, checkpoint=policy)
+        gen = torch.Generator(device="cuda").manual_seed(2)
+        target = model.forward(torch.randn((n,) * 3, device="cuda", generator=gen))
+        gen.manual_seed(1)
+        w = torch.randn((n,) * 3, device="cuda", generator=gen)
+        delta = model.forward(w)
+        dbar = 2.0 * (delta - target) / delta.numel()
+        model.backward(dbar)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            delta = model.forward(w)
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / reps
+        c._profile_reset()
+        c._profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g = model.backward(dbar)
+        torch.cuda.synchronize()
+        t_bwd = (time.perf_counter() - t0) / reps
+        prof = c._profile_get()
+        c._profile_enable(False)
+        top = sorted(prof, key=lambda r: -r[2])[:6]
+        res[policy] = {
+            "forward_ms": round(t_fwd * 1e3, 3),
+            "backward_ms": round(t_bwd * 1e3, 3),
+            "adjoint_forward_ratio": round(t_bwd / t_fwd, 3),
+            "grad_norm": float(g.norm()),
+            "backward_top_kernels_ms": {name: round(total / reps, 3)
+                                        for name, _, total, _ in top},
+        }
+        del model
+    res["adjoint_forward_ratio"] = res["store"]["adjoint_forward_ratio"]
+    res["checkpoint_bytes"] = {"positions": 12 * n**3 * (nstep + 1),
+                               "store_extra": nstep * (3 * 4 * nm * nm * 2 * ((nm // 2 + 1 + 15) // 16 * 16)
+                                                       + 16 * n**3)}
+    return res
+
+
 def cpu_baseline(n_p, nc, box, sub, workers):
     """Oracle port on the host: one COLA step with the four FFTs at full mesh
     size (FP32 scipy.fft, `workers` threads) and the particle loops (paint,
@@ -96,6 +151,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-sub", type=int, default=8)
+    ap.add_argument("--adjoint", default="on", choices=["on", "off"],
+                    help="also time the differentiable 128^3/256^3 run (adjoint/forward ratio)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -192,10 +249,15 @@ def main():
                           "frac": round(step_bytes / (T / args.steps) / 1e9 / PEAK_HBM_GBS, 4)},
         "kernels": kernels,
         "paint_variants_ms_final_state": paint_ms,
+        "adjoint": None,
         "cpu_baseline": None,
     }
+    if args.adjoint == "on":
+        del snap
+        print("[bench] adjoint 128^3/256^3", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_leg(fs, ps)
     if args.cpu_baseline == "on":
-        del p, snap
+        del p
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         try:
             workers = min(16, len(os.sched_getaffinity(0)))

@@ -33,12 +33,16 @@ def _load():
     lib = ctypes.CDLL(LIBRARY)
     vp, i32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     lib.fs2_last_error.restype = ctypes.c_char_p
-    lib.fs2_diff_create.argtypes = [i32, i32, f64, f64, f64, i32, vp, ctypes.POINTER(vp)]
+    lib.fs2_diff_create.argtypes = [i32, i32, f64, f64, f64, i32, vp, i32, ctypes.POINTER(vp)]
+    lib.fs2_diff_params.argtypes = [vp] + [ctypes.POINTER(i32)] * 3 + [
+        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(i32)]
+    lib.fs2_diff_set_paint_variant.argtypes = [vp, i32]
     lib.fs2_diff_destroy.argtypes = [vp]
     lib.fs2_diff_destroy.restype = None
     lib.fs2_diff_forward.argtypes = [vp, vp, vp]
     lib.fs2_diff_backward.argtypes = [vp, vp, vp]
-    for f in (lib.fs2_diff_create, lib.fs2_diff_forward, lib.fs2_diff_backward):
+    for f in (lib.fs2_diff_create, lib.fs2_diff_forward, lib.fs2_diff_backward,
+              lib.fs2_diff_params, lib.fs2_diff_set_paint_variant):
         f.restype = i32
     _lib = lib
     return lib
@@ -65,19 +69,40 @@ class ColaModel:
         boxsize: box length [1/h Mpc].
         a0, a1: initial and final scale factor; nstep COLA steps between.
         ps: fs.PowerSpectrum of the linear field.
+        checkpoint: 'auto' (store when it fits in half of the free device
+            memory), 'recompute' (positions only; each step's density and
+            force meshes are rebuilt in the backward sweep) or 'store'
+            (also keep every step's force meshes and particle bins).
     fs.cosmology.init must have been called.
     """
 
-    def __init__(self, nc, nc_pm, boxsize, a0, a1, nstep, ps):
+    _POLICY = {"auto": 0, "recompute": 1, "store": 2}
+    _VARIANT = {"sorted": 1, "global": 2}
+
+    def __init__(self, nc, nc_pm, boxsize, a0, a1, nstep, ps, checkpoint="auto"):
         lib = _load()
         self.nc, self.nc_pm, self.nstep = int(nc), int(nc_pm), int(nstep)
         self.boxsize, self.a0, self.a1 = float(boxsize), float(a0), float(a1)
         self._ps = ps   # keeps the power spectrum alive
         h = ctypes.c_void_p()
+        if checkpoint not in self._POLICY:
+            raise ValueError(f"checkpoint must be one of {sorted(self._POLICY)}")
         _check(lib.fs2_diff_create(self.nc, self.nc_pm, self.boxsize, self.a0, self.a1,
-                                   self.nstep, _power_pointer(ps), ctypes.byref(h)))
+                                   self.nstep, _power_pointer(ps), self._POLICY[checkpoint],
+                                   ctypes.byref(h)))
         self._h = h
         self.forward_count = 0
+        st = ctypes.c_int()
+        _check(lib.fs2_diff_params(h, None, None, None, None, ctypes.byref(st)))
+        self.stores_forces = bool(st.value)
+
+    def set_paint_variant(self, variant):
+        """'sorted' (tiled, deterministic; default) or 'global' (atomics and
+        gather kernels).  Invalidates the checkpoints of the last forward."""
+        if variant not in self._VARIANT:
+            raise ValueError(f"variant must be one of {sorted(self._VARIANT)}")
+        _check(_lib.fs2_diff_set_paint_variant(self._h, self._VARIANT[variant]))
+        self.forward_count += 1
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -278,6 +278,7 @@ void free_paint_scratch(PaintScratch& s) {
   dfree(s.cursor);
   dfree(s.block_sums);
   dfree(s.rec);
+  dfree(s.amax);
   s = PaintScratch();
 }
 
@@ -292,6 +293,7 @@ void ensure_paint_scratch(PaintScratch& s, size_t np, size_t nbins) {
   s.cursor = (unsigned*)dmalloc(nbins * sizeof(unsigned));
   s.block_sums = (unsigned*)dmalloc(4096 * sizeof(unsigned));
   s.rec = (float4*)dmalloc(cap * sizeof(float4));
+  s.amax = (unsigned*)dmalloc(4 * sizeof(unsigned));
   s.np = cap;
   s.nbins = nbins;
 }

@@ -1,5 +1,5 @@
-// Differentiable COLA run: forward with per-step position checkpoints and
-// the reverse-mode sweep (vector-Jacobian product) back to the white noise.
+// Differentiable COLA run: forward with per-step checkpoints and the
+// reverse-mode sweep (vector-Jacobian product) back to the white noise.
 //
 // No reference counterpart: fs2 has no adjoint.  The forward is the
 // reference's 'cola' pipeline -- 2LPT (py/lpt.cpp:100-225, 439-591) with
@@ -7,10 +7,13 @@
 // schedule of script/cola.py:37-44, forces by the PM chain
 // (py/pm.cpp:57-508), and the final CIC density.
 //
-// Memory is traded for time the MI355X way: positions of every step are
-// kept in HBM (12 bytes x particles x (nstep + 1)), so the backward sweep
-// only recomputes one density + force-mesh transform per step and never
-// re-runs the forward from the start.
+// Checkpoints.  Positions of every step are always kept (12 bytes x
+// particles x (nstep + 1)).  With FS2_DIFF_STORE the forward also keeps each
+// step's three force meshes and its particle bins, so the backward sweep
+// recomputes nothing; with FS2_DIFF_RECOMPUTE the backward re-bins and
+// re-transforms the density of each step from its positions.  The default
+// (FS2_DIFF_AUTO) stores when that fits in half of the free device memory,
+// which on a 288 GB part covers every configuration up to 512^3 / 1024^3.
 #include <cmath>
 #include <vector>
 
@@ -28,9 +31,10 @@ struct fs2_diff {
   };
   std::vector<Step> steps;
   int64_t np = 0;
-  bool tiled = true;          // tiled deterministic paint (else global atomics)
+  bool store = false;         // force meshes and bins of every step kept
+  int variant = FS2_PAINT_SORTED;   // FS2_PAINT_GLOBAL: atomic paint + gather kernels
   bool have_forward = false;
-  uint64_t forward_count = 0;
+  int shared_bins_step = -1;  // step whose positions the shared bins hold
   // device buffers
   double* amp = nullptr;
   float* noise = nullptr;     // padded copy of the last forward's noise (nl grid)
@@ -41,37 +45,47 @@ struct fs2_diff {
   float* dx1 = nullptr;
   float* dx2 = nullptr;
   float* f = nullptr;
-  float* xs = nullptr;        // checkpoints [nstep + 1][3][np]
+  float* xs = nullptr;        // positions [nstep + 1][3][np]
   float* xbar = nullptr;
   float* vbar = nullptr;
   float* dx1bar = nullptr;
   float* dx2bar = nullptr;
   float* fbar = nullptr;
   float* mesh = nullptr;      // nm grids
-  float* F[3] = {};
+  float* F[3] = {};           // force meshes (recompute mode)
   float* P[3] = {};
-  fs2::PaintScratch paint;
+  float* Fs = nullptr;        // store mode: [nstep][3] force meshes
+  fs2::PaintScratch paint;    // shared bins
+  std::vector<fs2::PaintScratch> bins;   // store mode: per-step rec / offsets
 
   float* x_at(int i) const { return xs + (size_t)i * 3 * np; }
+  float* force_mesh(int i, int a) const {
+    return store ? Fs + ((size_t)i * 3 + a) * fs2::grid_floats(nm) : F[a];
+  }
+  const fs2::PaintScratch& bins_of(int i) const {
+    return store && i < nstep ? bins[i] : paint;
+  }
 };
 
 namespace fs2 {
 namespace {
 
-std::vector<void*> buffers_of(fs2_diff* d) {
-  std::vector<void*> b = {d->amp, d->noise, d->S, d->v, d->dx1, d->dx2, d->f, d->xs, d->xbar,
-                          d->vbar, d->dx1bar, d->dx2bar, d->fbar, d->mesh};
-  for (float* g : d->G) b.push_back(g);
-  for (float* g : d->H) b.push_back(g);
-  for (float* g : d->F) b.push_back(g);
-  for (float* g : d->P) b.push_back(g);
-  return b;
-}
-
 void destroy(fs2_diff* d) {
   if (!d) return;
   (void)hipStreamSynchronize(stream());
-  for (void* p : buffers_of(d)) dfree(p);
+  for (void* p : {(void*)d->amp, (void*)d->noise, (void*)d->S, (void*)d->v, (void*)d->dx1,
+                  (void*)d->dx2, (void*)d->f, (void*)d->xs, (void*)d->xbar, (void*)d->vbar,
+                  (void*)d->dx1bar, (void*)d->dx2bar, (void*)d->fbar, (void*)d->mesh,
+                  (void*)d->Fs})
+    dfree(p);
+  for (float* g : d->G) dfree(g);
+  for (float* g : d->H) dfree(g);
+  for (float* g : d->F) dfree(g);
+  for (float* g : d->P) dfree(g);
+  for (PaintScratch& b : d->bins) {
+    dfree(b.rec);
+    dfree(b.offsets);
+  }
   free_paint_scratch(d->paint);
   delete d;
 }
@@ -86,15 +100,30 @@ void from_grid(float* dst, const float* grid, int n) {
                            n * sizeof(float), (size_t)n * n, hipMemcpyDeviceToDevice, stream()));
 }
 
-// CIC density fac * sum W - 1 of the positions at x into d->mesh; wraps x in place.
-void density(fs2_diff* d, float* x, bool wrap) {
-  const PosArgs pos{x, d->np, d->np};
-  if (d->tiled) {
-    paint_tiled(pos, d->mesh, d->nm, d->boxsize_f, d->fac, wrap, true, d->paint);
+bool tiled(const fs2_diff* d) { return d->variant == FS2_PAINT_SORTED; }
+
+// CIC density fac * sum W - 1 of the positions of step i into d->mesh,
+// binning them into bins (tiled variant); wrap = the positions are wrapped
+// into the box in place first.
+void density(fs2_diff* d, int i, const PaintScratch& bins, bool wrap) {
+  const PosArgs pos{d->x_at(i), d->np, d->np};
+  if (tiled(d)) {
+    paint_tiled(pos, d->mesh, d->nm, d->boxsize_f, d->fac, wrap, true, bins);
+    if (&bins == &d->paint) d->shared_bins_step = i;
   } else {
     fill_grid(d->mesh, d->nm, -1.0f);
     paint_atomic(pos, d->mesh, d->nm, d->boxsize_f, d->fac, wrap);
   }
+}
+
+void grad_readout(fs2_diff* d, int i, const float* rhobar, const float* const F[3],
+                  const float* fbar) {
+  if (tiled(d))
+    grad_readout_tiled(rhobar, d->fac, F, fbar, d->xbar, d->np, d->np, d->nm, d->boxsize_f,
+                       d->bins_of(i));
+  else
+    adj_grad_readout(d->x_at(i), d->np, d->np, rhobar, d->fac, F, fbar, d->xbar, d->nm,
+                     d->boxsize_f);
 }
 
 }  // namespace
@@ -105,19 +134,23 @@ using namespace fs2;
 extern "C" {
 
 int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1, int nstep,
-                    const fs2_power* ps, fs2_diff** out) {
+                    const fs2_power* ps, int checkpoint, fs2_diff** out) {
   return guard([&] {
     if (!out) fail(FS2_ERR_VALUE, "null output handle");
     *out = nullptr;
     if (!ps) fail(FS2_ERR_VALUE, "null power spectrum");
     if (nc_lpt < 4 || nc_lpt % 4 != 0 || !fft_supported(nc_lpt))
       fail(FS2_ERR_VALUE, "nc_lpt = %d not supported for 2LPT", nc_lpt);
-    if (nc_pm < 2 || !fft_supported(nc_pm))
+    if (nc_pm < 4 || !fft_supported(nc_pm))
       fail(FS2_ERR_VALUE, "nc_pm = %d is not a supported FFT size", nc_pm);
     if (boxsize <= 0) fail(FS2_ERR_VALUE, "boxsize must be positive");
     if (nstep < 0) fail(FS2_ERR_VALUE, "nstep must be >= 0");
     if (!(a0 > 0.0) || !(a1 >= a0)) fail(FS2_ERR_VALUE, "need 0 < a0 <= a1");
+    if (checkpoint != FS2_DIFF_AUTO && checkpoint != FS2_DIFF_RECOMPUTE &&
+        checkpoint != FS2_DIFF_STORE)
+      fail(FS2_ERR_VALUE, "unknown checkpoint policy %d", checkpoint);
     const double om = host::omega_m();   // cosmology must be initialised
+    (void)paint_tiled_bins(nc_pm);       // tile extents are powers of two for every FFT size
 
     fs2_diff* d = new fs2_diff();
     try {
@@ -140,7 +173,6 @@ int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1,
       for (int i = 0; i < nstep; ++i) {
         const double a_vel = a0 + (a1 - a0) / nstep * (i + 0.5);
         const double a_pos = a0 + (a1 - a0) / nstep * (i + 1.0);
-        fs2_diff::Step s;
         double kf, q1, q2, dt, da1, da2;
         int rc = fs2_cola_kick_coefficients(a_v, a_x, a_vel, &kf, &q1, &q2);
         if (rc) fail(rc, "%s", fs2_last_error());
@@ -148,6 +180,7 @@ int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1,
         rc = fs2_cola_drift_coefficients(a_x, a_v, a_pos, &dt, &da1, &da2);
         if (rc) fail(rc, "%s", fs2_last_error());
         a_x = a_pos;
+        fs2_diff::Step s;
         s.kf = (float)kf;
         s.q1 = (float)q1;
         s.q2 = (float)q2;
@@ -158,19 +191,26 @@ int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1,
         d->steps.push_back(s);
       }
 
-      try {
-        (void)paint_tiled_bins(nc_pm);
-      } catch (const Error&) {
-        d->tiled = false;   // tile extents must be powers of two; global atomics otherwise
+      const size_t lbytes = grid_floats(nc_lpt) * sizeof(float);
+      const size_t mbytes = grid_floats(nc_pm) * sizeof(float);
+      const size_t pbytes = 3 * (size_t)d->np * sizeof(float);
+      const size_t nbins = paint_tiled_bins(nc_pm);
+      const size_t store_bytes =
+          (size_t)nstep * (3 * mbytes + (size_t)d->np * sizeof(float4) + (nbins + 1) * 4);
+      if (checkpoint == FS2_DIFF_AUTO) {
+        size_t free_b = 0, total_b = 0;
+        FS2_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t base = 10 * lbytes + 10 * pbytes + pbytes * (size_t)(nstep + 1) +
+                            7 * mbytes + (size_t)d->np * 20;
+        d->store = base + store_bytes < free_b / 2;
+      } else {
+        d->store = checkpoint == FS2_DIFF_STORE;
       }
 
       const std::vector<double> amp = lpt_amp_table(nc_lpt, boxsize, ps);
       d->amp = (double*)dmalloc(amp.size() * sizeof(double));
       FS2_HIP(hipMemcpyAsync(d->amp, amp.data(), amp.size() * sizeof(double),
                              hipMemcpyHostToDevice, stream()));
-      const size_t lbytes = grid_floats(nc_lpt) * sizeof(float);
-      const size_t mbytes = grid_floats(nc_pm) * sizeof(float);
-      const size_t pbytes = 3 * (size_t)d->np * sizeof(float);
       d->noise = (float*)dmalloc(lbytes);
       d->S = (float*)dmalloc(lbytes);
       for (auto& g : d->G) g = (float*)dmalloc(lbytes);
@@ -180,9 +220,21 @@ int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1,
         *pp = (float*)dmalloc(pbytes);
       d->xs = (float*)dmalloc(pbytes * (size_t)(nstep + 1));
       d->mesh = (float*)dmalloc(mbytes);
-      for (auto& g : d->F) g = (float*)dmalloc(mbytes);
       for (auto& g : d->P) g = (float*)dmalloc(mbytes);
-      if (d->tiled) ensure_paint_scratch(d->paint, (size_t)d->np, paint_tiled_bins(nc_pm));
+      ensure_paint_scratch(d->paint, (size_t)d->np, nbins);
+      if (d->store) {
+        if (nstep > 0) d->Fs = (float*)dmalloc(3 * mbytes * (size_t)nstep);
+        for (int i = 0; i < nstep; ++i) {
+          PaintScratch b = d->paint;   // shares keys / counts / cursor / scan sums
+          b.rec = nullptr;
+          b.offsets = nullptr;
+          d->bins.push_back(b);
+          d->bins.back().rec = (float4*)dmalloc((size_t)d->np * sizeof(float4));
+          d->bins.back().offsets = (unsigned*)dmalloc((nbins + 1) * sizeof(unsigned));
+        }
+      } else {
+        for (auto& g : d->F) g = (float*)dmalloc(mbytes);
+      }
       // padding columns of the LPT grids stay defined
       FS2_HIP(hipMemsetAsync(d->noise, 0, lbytes, stream()));
       sync();
@@ -196,13 +248,25 @@ int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1,
 
 void fs2_diff_destroy(fs2_diff* d) { destroy(d); }
 
-int fs2_diff_params(const fs2_diff* d, int* nc_lpt, int* nc_pm, int* nstep, uint64_t* np) {
+int fs2_diff_params(const fs2_diff* d, int* nc_lpt, int* nc_pm, int* nstep, uint64_t* np,
+                    int* stores_forces) {
   return guard([&] {
     if (!d) fail(FS2_ERR_VALUE, "null handle");
     if (nc_lpt) *nc_lpt = d->nl;
     if (nc_pm) *nc_pm = d->nm;
     if (nstep) *nstep = d->nstep;
     if (np) *np = (uint64_t)d->np;
+    if (stores_forces) *stores_forces = d->store ? 1 : 0;
+  });
+}
+
+int fs2_diff_set_paint_variant(fs2_diff* d, int variant) {
+  return guard([&] {
+    if (!d) fail(FS2_ERR_VALUE, "null handle");
+    if (variant != FS2_PAINT_SORTED && variant != FS2_PAINT_GLOBAL)
+      fail(FS2_ERR_VALUE, "differentiable run supports the sorted and global paint variants");
+    d->variant = variant;
+    d->have_forward = false;   // checkpoints of the other variant are not reused
   });
 }
 
@@ -210,7 +274,7 @@ int fs2_diff_forward(fs2_diff* d, const float* noise, float* delta) {
   return guard([&] {
     if (!d) fail(FS2_ERR_VALUE, "null handle");
     if (!noise || !delta) fail(FS2_ERR_VALUE, "null device buffer");
-    const int nl = d->nl;
+    const int nl = d->nl, nm = d->nm;
     const int64_t np = d->np;
     d->have_forward = false;
 
@@ -228,24 +292,23 @@ int fs2_diff_forward(fs2_diff* d, const float* noise, float* delta) {
     const double m15om = -1.5 * host::omega_m();
     for (int i = 0; i < d->nstep; ++i) {
       const fs2_diff::Step& s = d->steps[i];
-      float* x = d->x_at(i);
-      density(d, x, true);   // wraps x_i in place: the checkpoint is the wrapped position
-      fft_force_meshes(d->mesh, d->F, d->nm, d->f1);
-      if (d->tiled)
-        readout_tiled(d->F[0], d->F[1], d->F[2], d->f, np, np, d->nm, d->boxsize_f, d->paint);
+      float* F[3] = {d->force_mesh(i, 0), d->force_mesh(i, 1), d->force_mesh(i, 2)};
+      density(d, i, d->bins_of(i), true);   // the checkpoint is the wrapped position
+      fft_force_meshes(d->mesh, F, nm, d->f1);
+      if (tiled(d))
+        readout_tiled(F[0], F[1], F[2], d->f, np, np, nm, d->boxsize_f, d->bins_of(i));
       else
-        readout3(PosArgs{x, np, np}, d->F[0], d->F[1], d->F[2], d->f, d->nm, d->boxsize_f);
+        readout3(PosArgs{d->x_at(i), np, np}, F[0], F[1], F[2], d->f, nm, d->boxsize_f);
       cola_kick(d->v, d->f, d->dx1, d->dx2, np, np, m15om, s.kf, s.q1, s.q2);
       float* xn = d->x_at(i + 1);
-      FS2_HIP(hipMemcpyAsync(xn, x, 3 * (size_t)np * sizeof(float), hipMemcpyDeviceToDevice,
-                             stream()));
+      FS2_HIP(hipMemcpyAsync(xn, d->x_at(i), 3 * (size_t)np * sizeof(float),
+                             hipMemcpyDeviceToDevice, stream()));
       cola_drift(xn, d->v, d->dx1, d->dx2, np, np, s.dt, s.da1, s.da2);
     }
-    density(d, d->x_at(d->nstep), true);
-    from_grid(delta, d->mesh, d->nm);
+    density(d, d->nstep, d->paint, true);
+    from_grid(delta, d->mesh, nm);
     sync();
     d->have_forward = true;
-    ++d->forward_count;
   });
 }
 
@@ -261,25 +324,30 @@ int fs2_diff_backward(fs2_diff* d, const float* delta_bar, float* noise_bar) {
       FS2_HIP(hipMemsetAsync(b, 0, pbytes, stream()));
 
     // final density: xbar_N = fac grad-readout of delta_bar
+    if (tiled(d) && d->shared_bins_step != d->nstep) density(d, d->nstep, d->paint, false);
     to_grid(d->P[0], delta_bar, nm);
-    adj_grad_readout(d->x_at(d->nstep), np, np, d->P[0], d->fac, nullptr, nullptr, d->xbar, nm,
-                     d->boxsize_f);
+    grad_readout(d, d->nstep, d->P[0], nullptr, nullptr);
 
     for (int i = d->nstep - 1; i >= 0; --i) {
       const fs2_diff::Step& s = d->steps[i];
-      float* x = d->x_at(i);
       adj_cola_step(d->xbar, d->vbar, d->dx1bar, d->dx2bar, d->fbar, n3, (float)s.dt, s.da1,
                     s.da2, s.c, s.q1, s.q2);
-      // force meshes at x_i, recomputed from the checkpoint
-      density(d, x, false);
-      fft_force_meshes(d->mesh, d->F, nm, d->f1);
+      float* F[3] = {d->force_mesh(i, 0), d->force_mesh(i, 1), d->force_mesh(i, 2)};
+      if (!d->store) {   // force meshes at x_i, recomputed from the checkpoint
+        density(d, i, d->paint, false);
+        fft_force_meshes(d->mesh, F, nm, d->f1);
+      }
       // rhobar = M^T (paint of fbar) = -sum_a M_a P_a
-      for (float* g : d->P) fill_grid(g, nm, 0.0f);
-      adj_paint_values3(x, d->fbar, np, np, d->P, nm, d->boxsize_f);
+      if (tiled(d)) {
+        paint_values3_tiled(d->fbar, np, np, d->P, nm, d->boxsize_f, d->bins_of(i));
+      } else {
+        for (float* g : d->P) fill_grid(g, nm, 0.0f);
+        adj_paint_values3(d->x_at(i), d->fbar, np, np, d->P, nm, d->boxsize_f);
+      }
       for (float* g : d->P) fft_r2c_3d(g, nm);
       adj_green_sum(d->P, nm, d->f1);
       fft_c2r_3d(d->P[0], nm);
-      adj_grad_readout(x, np, np, d->P[0], d->fac, d->F, d->fbar, d->xbar, nm, d->boxsize_f);
+      grad_readout(d, i, d->P[0], F, d->fbar);
     }
 
     // x_0 = q + D1 dx1 + D2 dx2 (v_0 = 0)

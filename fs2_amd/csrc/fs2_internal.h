@@ -126,6 +126,7 @@ struct PaintScratch {
   unsigned* cursor = nullptr;    // [nbins]
   unsigned* block_sums = nullptr;   // [4096] scan block totals
   float4* rec = nullptr;         // [np] (x, y, z, particle index) in bin order
+  unsigned* amax = nullptr;      // [4] per-field max |value| of the adjoint paint
   size_t np = 0, nbins = 0;
 };
 size_t paint_tiled_bins(int n);
@@ -137,6 +138,16 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
 // paint_tiled call (positions unchanged since); forces land at f[i + a*fstride].
 void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, int64_t fstride,
                    int64_t np, int n, float boxsize, const PaintScratch& s);
+// Adjoint kernels over the bins of the last paint_tiled call (positions
+// unchanged since).  paint_values3_tiled: out_a(c) = sum_p W(x_p - c) v_a(p)
+// for the value planes v_a = vals[i + a*vstride], int64 fixed-point sums, every
+// cell written.  grad_readout_tiled: xbar_p += sum_c grad W(x_p - c)
+// [fac rhobar(c) + sum_a fbar_a(p) F_a(c)] (F may be null: rhobar term only).
+void paint_values3_tiled(const float* vals, int64_t vstride, int64_t np, float* const out[3],
+                         int n, float boxsize, const PaintScratch& s);
+void grad_readout_tiled(const float* rhobar, float fac, const float* const F[3],
+                        const float* fbar, float* xbar, int64_t stride, int64_t np, int n,
+                        float boxsize, const PaintScratch& s);
 void power_spectrum(const float* delta_k, int n, double boxsize, int nbins,
                     double dk_bin, double* k, double* P, double* nmodes);
 

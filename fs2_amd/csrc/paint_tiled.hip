@@ -226,32 +226,17 @@ __global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t n
   }
 }
 
-// K4: one workgroup per tile.  ACC is float (LDS float atomics) or
-// unsigned long long (int64 fixed point, deterministic).
-template <typename ACC>
-__global__ void __launch_bounds__(kBlock)
-k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
-             TileGeom g, float fac, float* __restrict__ grid) {
-  extern __shared__ unsigned char smem[];
-  ACC* acc = (ACC*)smem;
-  const int tcells = g.tx * g.ty * g.tz;
-  for (int c = threadIdx.x; c < tcells; c += blockDim.x) acc[c] = (ACC)0;
-  __syncthreads();
-
-  const int tile = blockIdx.x;
-  const int Tz = tile % g.ntz;
-  const int Ty = (tile / g.ntz) % g.nty;
-  const int Tx = tile / (g.ntz * g.nty);
-  const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
-
-  // Source tiles S = T - d, d in {0,1}^3 (periodic).  A particle of S
-  // reaches T only by spilling along every axis where S != T, so S
-  // contributes the classes c with (c & need) == need.  All 8 x 9 bin
-  // offsets are loaded at once; the needed class runs are merged into at
-  // most 15 contiguous ranges, which all threads then stride over.
-  __shared__ unsigned offs[8][9];
-  __shared__ unsigned rb[16], rpre[17];
-  __shared__ int nranges;
+// Source tiles S = T - d, d in {0,1}^3 (periodic).  A particle of S
+// reaches T only by spilling along every axis where S != T, so S
+// contributes the classes c with (c & need) == need.  All 8 x 9 bin
+// offsets are loaded at once; the needed class runs are merged into at
+// most 15 contiguous ranges [rb[r], rb[r] + rpre[r+1] - rpre[r]) of the
+// record array, which all threads of the block then stride over.
+// Block-cooperative; ends with a barrier.
+__device__ __forceinline__ void tile_source_ranges(const unsigned* __restrict__ offsets,
+                                                   const TileGeom& g, int Tx, int Ty, int Tz,
+                                                   unsigned (*offs)[9], unsigned* rb,
+                                                   unsigned* rpre, int* nranges) {
   if (threadIdx.x < 72) {
     const int d = threadIdx.x / 9, cl = threadIdx.x % 9;
     const int Sx = (Tx - ((d >> 2) & 1) + g.ntx) % g.ntx;
@@ -289,9 +274,33 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
       }
     }
     if (nr == 0) rpre[0] = 0;
-    nranges = nr;
+    *nranges = nr;
   }
   __syncthreads();
+}
+
+// K4: one workgroup per tile.  ACC is float (LDS float atomics) or
+// unsigned long long (int64 fixed point, deterministic).
+template <typename ACC>
+__global__ void __launch_bounds__(kBlock)
+k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
+             TileGeom g, float fac, float* __restrict__ grid) {
+  extern __shared__ unsigned char smem[];
+  ACC* acc = (ACC*)smem;
+  const int tcells = g.tx * g.ty * g.tz;
+  for (int c = threadIdx.x; c < tcells; c += blockDim.x) acc[c] = (ACC)0;
+  __syncthreads();
+
+  const int tile = blockIdx.x;
+  const int Tz = tile % g.ntz;
+  const int Ty = (tile / g.ntz) % g.nty;
+  const int Tx = tile / (g.ntz * g.nty);
+  const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
+
+  __shared__ unsigned offs[8][9];
+  __shared__ unsigned rb[16], rpre[17];
+  __shared__ int nranges;
+  tile_source_ranges(offsets, g, Tx, Ty, Tz, offs, rb, rpre, &nranges);
   const int nr = nranges;
   const unsigned total = rpre[nr];
   int r = 0;
@@ -400,6 +409,200 @@ k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offs
   }
 }
 
+// ---- adjoint kernels over the same bins ---------------------------------------------
+// Max |v| of each of three value planes, as ordered float bits (|v| >= 0, so
+// the unsigned order of the bits is the float order); amax cleared to 0.
+// Grid-stride with a block reduction: one atomic per block and field.
+__global__ void __launch_bounds__(kBlock)
+k_absmax3(const float* __restrict__ v, int64_t stride, int64_t np, unsigned* __restrict__ amax) {
+  __shared__ unsigned red[3][kBlock / 64];
+  unsigned m[3] = {0u, 0u, 0u};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np;
+       i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) m[a] = max(m[a], __float_as_uint(fabsf(v[i + a * stride])));
+  }
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m[a] = max(m[a], (unsigned)__shfl_xor((int)m[a], off));
+    if (__lane_id() == 0) red[a][wave] = m[a];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    unsigned r = 0u;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) r = max(r, red[threadIdx.x][w]);
+    if (r) atomicMax(amax + threadIdx.x, r);
+  }
+}
+
+// Fixed-point scale of one value plane: every cell sum is bounded by
+// sum_p |v_p| <= np * amax < 2^e, so scale = 2^(62 - e) keeps int64 sums
+// exact; a term's rounding error is 2^(e - 62), i.e. np * amax * 2^-62.
+__device__ __forceinline__ float value_scale(unsigned amax_bits, float npf) {
+  const float amax = __uint_as_float(amax_bits);
+  if (!(amax > 0.f)) return 1.f;
+  const float bound = amax * npf;
+  if (!(bound < 3.0e38f)) return 0x1p-66f;   // inf / nan values: results are not finite anyway
+  int e;
+  frexpf(bound, &e);
+  return ldexpf(1.f, 62 - e);
+}
+
+// Transpose of the three-field readout: out_a(c) = sum_p W(x_p - c) v_a(p),
+// owner-computes per tile like k_tile_paint, with int64 fixed-point LDS sums
+// (bitwise reproducible).  Every cell of the mesh is written.
+__global__ void __launch_bounds__(512)
+k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
+                     TileGeom g, const float* __restrict__ vals, int64_t vstride, float npf,
+                     const unsigned* __restrict__ amax, float* __restrict__ o0,
+                     float* __restrict__ o1, float* __restrict__ o2) {
+  extern __shared__ unsigned char smem[];
+  unsigned long long* acc = (unsigned long long*)smem;
+  const int tcells = g.tx * g.ty * g.tz;
+  for (int c = threadIdx.x; c < 3 * tcells; c += blockDim.x) acc[c] = 0ull;
+
+  const int tile = blockIdx.x;
+  const int Tz = tile % g.ntz;
+  const int Ty = (tile / g.ntz) % g.nty;
+  const int Tx = tile / (g.ntz * g.nty);
+  const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
+  float sc[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) sc[a] = value_scale(amax[a], npf);
+
+  __shared__ unsigned offs[8][9];
+  __shared__ unsigned rb[16], rpre[17];
+  __shared__ int nranges;
+  tile_source_ranges(offsets, g, Tx, Ty, Tz, offs, rb, rpre, &nranges);
+  const int nr = nranges;
+  const unsigned total = rpre[nr];
+  int r = 0;
+  for (unsigned v = threadIdx.x; v < total; v += blockDim.x) {
+    while (v >= rpre[r + 1]) ++r;
+    const unsigned j = rb[r] + (v - rpre[r]);
+    const float4 r4 = rec[j];
+    const unsigned i = __float_as_uint(r4.w);
+    float val[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) val[a] = vals[i + a * vstride];
+    const Cic1 cx = cic1(r4.x, g.dx_inv, g.n);
+    const Cic1 cy = cic1(r4.y, g.dx_inv, g.n);
+    const Cic1 cz = cic1(r4.z, g.dx_inv, g.n);
+    const int lx[2] = {cx.i0 - x0, cx.i1 - x0};
+    const int ly[2] = {cy.i0 - y0, cy.i1 - y0};
+    const int lz[2] = {cz.i0 - z0, cz.i1 - z0};
+    const float wx[2] = {cx.w0, cx.w1};
+    const float wy[2] = {cy.w0, cy.w1};
+    const float wz[2] = {cz.w0, cz.w1};
+#pragma unroll
+    for (int ex = 0; ex < 2; ++ex) {
+      if ((unsigned)lx[ex] >= (unsigned)g.tx) continue;
+#pragma unroll
+      for (int ey = 0; ey < 2; ++ey) {
+        if ((unsigned)ly[ey] >= (unsigned)g.ty) continue;
+#pragma unroll
+        for (int ez = 0; ez < 2; ++ez) {
+          if ((unsigned)lz[ez] >= (unsigned)g.tz) continue;
+          const float w = wx[ex] * wy[ey] * wz[ez];
+          const int li = (((lx[ex] << g.ly) + ly[ey]) << g.lz) + lz[ez];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const long long q = (long long)rintf(w * val[a] * sc[a]);
+            atomicAdd(&acc[a * tcells + li], (unsigned long long)q);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  float* outs[3] = {o0, o1, o2};
+  for (int c = threadIdx.x; c < tcells; c += blockDim.x) {
+    const int lz = c & (g.tz - 1);
+    const int ly = (c >> g.lz) & (g.ty - 1);
+    const int lx = c >> (g.lz + g.ly);
+    const int64_t gi = ((int64_t)(x0 + lx) * g.n + (y0 + ly)) * g.ncz + (z0 + lz);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      outs[a][gi] = (float)((double)(long long)acc[a * tcells + c] / (double)sc[a]);
+  }
+}
+
+// Gradient readout at the tile's own particles:
+//   xbar_p += sum_c grad W(x_p - c) [fac rhobar(c) + sum_a fbar_a(p) F_a(c)]
+// with rhobar (and, if FORCES, the three force meshes) staged over the tile
+// and its +1 halo in LDS.  Same FP32 expression order as the gather kernel
+// of adjoint_kernels.hip, so the two agree bitwise.
+template <bool FORCES>
+__global__ void __launch_bounds__(512)
+k_tile_grad_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
+                    TileGeom g, const float* __restrict__ rhobar, float fac,
+                    const float* __restrict__ m0, const float* __restrict__ m1,
+                    const float* __restrict__ m2, const float* __restrict__ fbar,
+                    float* __restrict__ xbar, int64_t stride) {
+  extern __shared__ float hs[];
+  constexpr int NM = FORCES ? 4 : 1;
+  const int hx = g.tx + 1, hy = g.ty + 1, hz = g.tz + 1;
+  const int hcells = hx * hy * hz;
+  const int tile = blockIdx.x;
+  const int Tz = tile % g.ntz;
+  const int Ty = (tile / g.ntz) % g.nty;
+  const int Tx = tile / (g.ntz * g.nty);
+  const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
+  const float* ms[4] = {rhobar, m0, m1, m2};
+  for (int c = threadIdx.x; c < hcells; c += blockDim.x) {
+    const int lz = c % hz;
+    const int r = c / hz;
+    const int ly = r % hy, lx = r / hy;
+    int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    if (gx >= g.n) gx -= g.n;
+    if (gy >= g.n) gy -= g.n;
+    if (gz >= g.n) gz -= g.n;
+    const int64_t gi = ((int64_t)gx * g.n + gy) * g.ncz + gz;
+#pragma unroll
+    for (int a = 0; a < NM; ++a) hs[a * hcells + c] = ms[a][gi];
+  }
+  __syncthreads();
+  const float dxi = g.dx_inv;
+  const unsigned b = offsets[(unsigned)tile * 8u], e = offsets[(unsigned)tile * 8u + 8u];
+  for (unsigned j = b + threadIdx.x; j < e; j += blockDim.x) {
+    const float4 r4 = rec[j];
+    const unsigned i = __float_as_uint(r4.w);
+    const Cic1 cx = cic1(r4.x, dxi, g.n);
+    const Cic1 cy = cic1(r4.y, dxi, g.n);
+    const Cic1 cz = cic1(r4.z, dxi, g.n);
+    float fb[3] = {0.f, 0.f, 0.f};
+    if constexpr (FORCES) {
+      fb[0] = fbar[i];
+      fb[1] = fbar[i + stride];
+      fb[2] = fbar[i + 2 * stride];
+    }
+    const int lx = cx.i0 - x0, ly = cy.i0 - y0, lz = cz.i0 - z0;
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+    for (int ex = 0; ex < 2; ++ex)
+#pragma unroll
+      for (int ey = 0; ey < 2; ++ey)
+#pragma unroll
+        for (int ez = 0; ez < 2; ++ez) {
+          const int c = ((lx + ex) * hy + (ly + ey)) * hz + (lz + ez);
+          float S = fac * hs[c];
+          if constexpr (FORCES)
+            S += fb[0] * hs[hcells + c] + fb[1] * hs[2 * hcells + c] + fb[2] * hs[3 * hcells + c];
+          const float wx = ex ? cx.w1 : cx.w0, wy = ey ? cy.w1 : cy.w0, wz = ez ? cz.w1 : cz.w0;
+          const float sx = ex ? dxi : -dxi, sy = ey ? dxi : -dxi, sz = ez ? dxi : -dxi;
+          gx += sx * wy * wz * S;
+          gy += wx * sy * wz * S;
+          gz += wx * wy * sz * S;
+        }
+    xbar[i] += gx;
+    xbar[i + stride] += gy;
+    xbar[i + 2 * stride] += gz;
+  }
+}
+
 TileGeom make_geom(int n, float boxsize) {
   TileGeom g;
   g.n = n;
@@ -476,6 +679,43 @@ void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, 
   const double bytes = (double)np * 24.0 + 3.0 * 4.0 * (double)n * n * n;
   launch("readout_tile", bytes, k_tile_readout, dim3((unsigned)ntiles), dim3(512), lds,
          (const float4*)s.rec, (const unsigned*)s.offsets, g, g0, g1, g2, f, fstride);
+}
+
+void paint_values3_tiled(const float* vals, int64_t vstride, int64_t np, float* const out[3],
+                         int n, float boxsize, const PaintScratch& s) {
+  const TileGeom g = make_geom(n, boxsize);
+  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
+  if ((size_t)(ntiles * 8) > s.nbins || (size_t)np > s.np)
+    fail(FS2_ERR_RUNTIME, "paint scratch does not hold bins for this mesh / particle count");
+  launch("adj_absmax_clear", 12.0, k_fill_u32, dim3(1), dim3(kBlock), 0, s.amax, (int64_t)3, 0u);
+  if (np > 0)
+    launch("adj_absmax3", (double)np * 12.0, k_absmax3,
+           dim3(std::min<unsigned>(blocks_for(np), 1024u)), dim3(kBlock), 0, vals, vstride, np,
+           s.amax);
+  const int tcells = g.tx * g.ty * g.tz;
+  launch("adj_paint_tile3", (double)np * 28.0 + 12.0 * (double)n * n * n, k_tile_paint_values3,
+         dim3((unsigned)ntiles), dim3(512), 3 * tcells * sizeof(unsigned long long),
+         (const float4*)s.rec, (const unsigned*)s.offsets, g, vals, vstride, (float)np,
+         (const unsigned*)s.amax, out[0], out[1], out[2]);
+}
+
+void grad_readout_tiled(const float* rhobar, float fac, const float* const F[3],
+                        const float* fbar, float* xbar, int64_t stride, int64_t np, int n,
+                        float boxsize, const PaintScratch& s) {
+  const TileGeom g = make_geom(n, boxsize);
+  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
+  const size_t hcells = (size_t)(g.tx + 1) * (g.ty + 1) * (g.tz + 1);
+  if (F) {
+    launch("adj_grad_tile4", (double)np * 40.0 + 16.0 * (double)n * n * n,
+           k_tile_grad_readout<true>, dim3((unsigned)ntiles), dim3(512), 4 * hcells * sizeof(float),
+           (const float4*)s.rec, (const unsigned*)s.offsets, g, rhobar, fac, F[0], F[1], F[2],
+           fbar, xbar, stride);
+  } else {
+    launch("adj_grad_tile1", (double)np * 28.0 + 4.0 * (double)n * n * n,
+           k_tile_grad_readout<false>, dim3((unsigned)ntiles), dim3(512), hcells * sizeof(float),
+           (const float4*)s.rec, (const unsigned*)s.offsets, g, rhobar, fac, nullptr, nullptr,
+           nullptr, nullptr, xbar, stride);
+  }
 }
 
 }  // namespace fs2

@@ -248,14 +248,22 @@ int fs2_lpt_white_noise(int nc, uint64_t seed, float* out);
  * stream (fs2_get_stream); both calls synchronize that stream on return.
  *   noise:     nc_lpt^3 FP32, [ix][iy][iz] contiguous (the white-noise field)
  *   delta:     nc_pm^3 FP32, the final density contrast
- * The handle keeps the positions of every step of the last forward
- * (12 x nc_lpt^3 x (nstep + 1) bytes); fs2_diff_backward differentiates that
- * forward and fails with FS2_ERR_RUNTIME if none ran.  Requires
- * fs2_cosmology_init. */
+ * The handle keeps the checkpoints of its last forward; fs2_diff_backward
+ * differentiates that forward (any number of times) and fails with
+ * FS2_ERR_RUNTIME if none ran.  Requires fs2_cosmology_init. */
+enum {
+  FS2_DIFF_AUTO = 0,       /* store when it fits in half of the free device memory */
+  FS2_DIFF_RECOMPUTE = 1,  /* keep positions only; re-bin and re-transform per step */
+  FS2_DIFF_STORE = 2       /* also keep every step's force meshes and particle bins */
+};
 int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1, int nstep,
-                    const fs2_power* ps, fs2_diff** out);
+                    const fs2_power* ps, int checkpoint, fs2_diff** out);
 void fs2_diff_destroy(fs2_diff* d);
-int fs2_diff_params(const fs2_diff* d, int* nc_lpt, int* nc_pm, int* nstep, uint64_t* np);
+int fs2_diff_params(const fs2_diff* d, int* nc_lpt, int* nc_pm, int* nstep, uint64_t* np,
+                    int* stores_forces);
+/* FS2_PAINT_SORTED (default: tiled deterministic paint and tiled adjoint
+ * kernels) or FS2_PAINT_GLOBAL (global atomics and gather kernels). */
+int fs2_diff_set_paint_variant(fs2_diff* d, int variant);
 int fs2_diff_forward(fs2_diff* d, const float* noise, float* delta);
 /* delta_bar: d loss / d delta (nc_pm^3); noise_bar: d loss / d noise (nc_lpt^3). */
 int fs2_diff_backward(fs2_diff* d, const float* delta_bar, float* noise_bar);
