@@ -92,6 +92,51 @@ def adjoint_leg(fs, ps, reps=5):
     return res
 
 
+def batch_leg(fs, ps, reps=5):
+    """Batched ensemble: 96 realisations with distinct seeds, 32 x (32^3 p,
+    64^3 mesh, box 32), 32 x (48^3 p, 96^3 mesh, box 48), 32 x (64^3 p, 128^3
+    mesh, box 64) Mpc/h, 2LPT at a = 0.1, 10 COLA steps to a = 1, all in one
+    hipGraph (one stream per size class) with autotuned paint tile shapes.
+    value = total particles * 10 / T(one graph replay), ICs restored before
+    each replay outside the timed region."""
+    from fs2_amd.batch import Ensemble
+    classes = [(32, 64, 32, 32.0), (48, 96, 32, 48.0), (64, 128, 32, 64.0)]
+    nstep = 10
+    ens = Ensemble(classes)
+    ens.init_lpt(ps, 0.1, range(1, ens.nbox_total + 1))
+    tuned = ens.autotune(reps=3)
+    ens.set_schedule(0.1, 1.0, nstep)
+    ens.capture()
+
+    def timed(graph):
+        ts = []
+        for _ in range(reps):
+            ens.reset()
+            ens.synchronize()
+            t0 = time.perf_counter()
+            ens.run(graph=graph)
+            ens.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        return ts[len(ts) // 2]
+
+    timed(True)   # warm-up replay
+    t_graph = timed(True)
+    t_eager = timed(False)
+    return {
+        "workload": "96 boxes: 32 x 32^3 p/64^3 mesh/32 Mpc/h + 32 x 48^3 p/96^3/48 + "
+                    "32 x 64^3 p/128^3/64, 10 COLA steps a=0.1->1, one hipGraph",
+        "particles": ens.np_total,
+        "value": ens.np_total * nstep / t_graph,
+        "unit": "particle-steps/s",
+        "ms_graph": round(t_graph * 1e3, 3),
+        "ms_eager": round(t_eager * 1e3, 3),
+        "eager_value": ens.np_total * nstep / t_eager,
+        "tiles": {f"{c[0]}^3/{c[1]}^3": {"tile": list(t), "bin_paint_readout_ms": round(ms, 4)}
+                  for c, (t, ms) in zip(classes, tuned)},
+    }
+
+
 def cpu_baseline(n_p, nc, box, sub, workers):
     """Oracle port on the host: one COLA step with the four FFTs at full mesh
     size (FP32 scipy.fft, `workers` threads) and the particle loops (paint,
@@ -153,6 +198,8 @@ def main():
     ap.add_argument("--cpu-sub", type=int, default=8)
     ap.add_argument("--adjoint", default="on", choices=["on", "off"],
                     help="also time the differentiable 128^3/256^3 run (adjoint/forward ratio)")
+    ap.add_argument("--batch", default="on", choices=["on", "off"],
+                    help="also time the 96-box batched ensemble in one hipGraph")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -250,12 +297,16 @@ def main():
         "kernels": kernels,
         "paint_variants_ms_final_state": paint_ms,
         "adjoint": None,
+        "batch": None,
         "cpu_baseline": None,
     }
     if args.adjoint == "on":
         del snap
         print("[bench] adjoint 128^3/256^3", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(fs, ps)
+    if args.batch == "on":
+        print("[bench] batched ensemble (96 boxes, hipGraph)", file=sys.stderr, flush=True)
+        out["batch"] = batch_leg(fs, ps)
     if args.cpu_baseline == "on":
         del p
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

@@ -99,10 +99,23 @@ void device_init() {
   g_inited = true;
 }
 
+namespace {
+thread_local hipStream_t t_stream_override = nullptr;
+}
+
 hipStream_t stream() {
+  if (t_stream_override) return t_stream_override;
   if (!g_inited) device_init();
   return g_stream;
 }
+
+hipStream_t library_stream() {
+  if (!g_inited) device_init();
+  return g_stream;
+}
+
+StreamScope::StreamScope(hipStream_t s) : prev_(t_stream_override) { t_stream_override = s; }
+StreamScope::~StreamScope() { t_stream_override = prev_; }
 
 void* dmalloc(size_t bytes) {
   device_init();
@@ -196,6 +209,9 @@ void prof_end(int token) {
   if (token < 0 || !g_prof) return;
   FS2_HIP(hipEventRecord(g_pending[token].b, stream()));
 }
+
+bool profiling_enabled() { return g_prof; }
+void set_profiling(bool on) { g_prof = on; }
 
 // ---- helpers ----------------------------------------------------------------------------------
 namespace {

@@ -239,12 +239,14 @@ template <int N, int P, bool INV, bool GREEN>
 __global__ void __launch_bounds__(Plan<N>::T * P)
 k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
              int64_t outer_stride, int nchunks, int nblocks, const float2* __restrict__ tw,
-             GreenArg g) {
+             GreenArg g, int64_t bstride) {
   constexpr int T = Plan<N>::T;
   using L = RowsLayout<P>;
   __shared__ float2 s[L::rows(N) * P];
   __shared__ float2 stw[N];
   stage_twiddles<N>(stw, tw);
+  src += blockIdx.y * bstride;   // mesh of this block's box (batched launches)
+  dst += blockIdx.y * bstride;
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
   const int bid = xcd_remap(blockIdx.x, nblocks);
@@ -283,12 +285,17 @@ template <int N, int P>
 __global__ void __launch_bounds__(Plan<N>::T * P)
 k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int ncols,
                int64_t elem_stride, int64_t outer_stride, int nchunks, int nblocks,
-               const float2* __restrict__ tw, float f1) {
+               const float2* __restrict__ tw, float f1, int64_t bstride) {
   constexpr int T = Plan<N>::T;
   using L = RowsLayout<P>;
   __shared__ float2 s[L::rows(N) * P];
   __shared__ float2 stw[N];
   stage_twiddles<N>(stw, tw);
+  const int64_t boff = blockIdx.y * bstride;   // mesh of this block's box (batched launches)
+  src += boff;
+  dst0 += boff;
+  dst1 += boff;
+  dst2 += boff;
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
   const int bid = xcd_remap(blockIdx.x, nblocks);
@@ -466,70 +473,75 @@ struct Impl {
 
   static double grid_bytes() { return (double)n * n * (n / 2 + 1) * 8.0; }
 
+  // nb meshes of grid_floats(n) floats each, back to back: the z passes
+  // treat them as nb*n*n rows, the pencil passes take the box from blockIdx.y.
+  static int64_t bstride() { return (int64_t)grid_floats(n) / 2; }   // in float2
+
   template <bool INV, bool GREEN>
   static void pencil(const char* name, const float2* src, float2* dst, int64_t elem_stride,
-                     int64_t outer_stride, GreenArg g) {
+                     int64_t outer_stride, GreenArg g, int nb) {
     const int nh = n / 2 + 1;
     const int nchunks = (nh + P - 1) / P;
     const int nblocks = n * nchunks;
-    launch(name, 2.0 * grid_bytes(), k_fft_pencil<N, P, INV, GREEN>, dim3(nblocks),
+    launch(name, 2.0 * grid_bytes() * nb, k_fft_pencil<N, P, INV, GREEN>, dim3(nblocks, nb),
            dim3(Plan<N>::T * P), 0, src, dst, nh, elem_stride, outer_stride, nchunks, nblocks,
-           twiddles(N), g);
+           twiddles(N), g, bstride());
   }
 
-  static void z_r2c(float* grid) {
-    const int64_t nrows = (int64_t)n * n;
-    const double bytes = (double)n * n * n * 4.0 + grid_bytes();
+  static void z_r2c(float* grid, int nb) {
+    const int64_t nrows = (int64_t)n * n * nb;
+    const double bytes = ((double)n * n * n * 4.0 + grid_bytes()) * nb;
     launch("fft_z_r2c", bytes, k_fft_z_r2c<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
            dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
            twiddles(N));
   }
 
-  static void z_c2r(float* grid) {
-    const int64_t nrows = (int64_t)n * n;
-    const double bytes = (double)n * n * n * 4.0 + grid_bytes();
+  static void z_c2r(float* grid, int nb) {
+    const int64_t nrows = (int64_t)n * n * nb;
+    const double bytes = ((double)n * n * n * 4.0 + grid_bytes()) * nb;
     launch("fft_z_c2r", bytes, k_fft_z_c2r<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
            dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
            twiddles(N));
   }
 
-  static void r2c(float* grid) {
+  static void r2c(float* grid, int nb) {
     const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)grid;
     GreenArg g{0, n, 0.f};
-    z_r2c(grid);
-    pencil<false, false>("fft_y_fwd", c, c, nh, (int64_t)n * nh, g);
-    pencil<false, false>("fft_x_fwd", c, c, (int64_t)n * nh, nh, g);
+    z_r2c(grid, nb);
+    pencil<false, false>("fft_y_fwd", c, c, nh, (int64_t)n * nh, g, nb);
+    pencil<false, false>("fft_x_fwd", c, c, (int64_t)n * nh, nh, g, nb);
   }
 
-  static void c2r(float* grid) {
+  static void c2r(float* grid, int nb) {
     const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)grid;
     GreenArg g{0, n, 0.f};
-    pencil<true, false>("fft_x_inv", c, c, (int64_t)n * nh, nh, g);
-    pencil<true, false>("fft_y_inv", c, c, nh, (int64_t)n * nh, g);
-    z_c2r(grid);
+    pencil<true, false>("fft_x_inv", c, c, (int64_t)n * nh, nh, g, nb);
+    pencil<true, false>("fft_y_inv", c, c, nh, (int64_t)n * nh, g, nb);
+    z_c2r(grid, nb);
   }
 
   // density -> three real force meshes (pm.cpp:431-508)
-  static void force_meshes(float* mesh, float* const out[3], float f1) {
+  static void force_meshes(float* mesh, float* const out[3], float f1, int nb) {
     const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)mesh;
     GreenArg g{0, n, 0.f};
-    z_r2c(mesh);
-    pencil<false, false>("fft_y_fwd", c, c, nh, (int64_t)n * nh, g);
+    z_r2c(mesh, nb);
+    pencil<false, false>("fft_y_fwd", c, c, nh, (int64_t)n * nh, g, nb);
     {
       const int ncols = n / 2 + 1;
       const int nchunks = (ncols + P - 1) / P;
       const int nblocks = n * nchunks;
-      launch("fft_x_fwd_green3", 4.0 * grid_bytes(), k_fft_x_green3<N, P>, dim3(nblocks),
-             dim3(Plan<N>::T * P), 0, (const float2*)c, (float2*)out[0], (float2*)out[1],
-             (float2*)out[2], ncols, (int64_t)n * nh, nh, nchunks, nblocks, twiddles(N), f1);
+      launch("fft_x_fwd_green3", 4.0 * grid_bytes() * nb, k_fft_x_green3<N, P>,
+             dim3(nblocks, nb), dim3(Plan<N>::T * P), 0, (const float2*)c, (float2*)out[0],
+             (float2*)out[1], (float2*)out[2], ncols, (int64_t)n * nh, nh, nchunks, nblocks,
+             twiddles(N), f1, bstride());
     }
     for (int a = 0; a < 3; ++a) {
       float2* o = (float2*)out[a];
-      pencil<true, false>("fft_y_inv", o, o, nh, (int64_t)n * nh, g);
-      z_c2r(out[a]);
+      pencil<true, false>("fft_y_inv", o, o, nh, (int64_t)n * nh, g, nb);
+      z_c2r(out[a], nb);
     }
   }
 
@@ -537,9 +549,9 @@ struct Impl {
     const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)dst;
     GreenArg g{axis, n, f1};
-    pencil<true, true>("fft_x_inv_green", (const float2*)dk, c, (int64_t)n * nh, nh, g);
-    pencil<true, false>("fft_y_inv", c, c, nh, (int64_t)n * nh, g);
-    z_c2r(dst);
+    pencil<true, true>("fft_x_inv_green", (const float2*)dk, c, (int64_t)n * nh, nh, g, 1);
+    pencil<true, false>("fft_y_inv", c, c, nh, (int64_t)n * nh, g, 1);
+    z_c2r(dst, 1);
   }
 };
 
@@ -576,16 +588,16 @@ bool fft_supported(int n) {
   }
 }
 
-void fft_r2c_3d(float* grid, int n) {
-  dispatch(n, [&](auto impl) { decltype(impl)::r2c(grid); });
+void fft_r2c_3d(float* grid, int n, int nb) {
+  dispatch(n, [&](auto impl) { decltype(impl)::r2c(grid, nb); });
 }
 
-void fft_c2r_3d(float* grid, int n) {
-  dispatch(n, [&](auto impl) { decltype(impl)::c2r(grid); });
+void fft_c2r_3d(float* grid, int n, int nb) {
+  dispatch(n, [&](auto impl) { decltype(impl)::c2r(grid, nb); });
 }
 
-void fft_force_meshes(float* mesh, float* const out[3], int n, float f1) {
-  dispatch(n, [&](auto impl) { decltype(impl)::force_meshes(mesh, out, f1); });
+void fft_force_meshes(float* mesh, float* const out[3], int n, float f1, int nb) {
+  dispatch(n, [&](auto impl) { decltype(impl)::force_meshes(mesh, out, f1, nb); });
 }
 
 void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f1) {

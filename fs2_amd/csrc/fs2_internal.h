@@ -57,7 +57,20 @@ int guard(F&& f) {
 
 // ---- runtime --------------------------------------------------------------
 void device_init();                 // throws Error(FS2_ERR_DEVICE) without a GPU
+// The stream every launch and copy of this thread goes to: the library
+// stream, or the stream of the innermost live StreamScope.
 hipStream_t stream();
+hipStream_t library_stream();
+class StreamScope {
+ public:
+  explicit StreamScope(hipStream_t s);
+  ~StreamScope();
+  StreamScope(const StreamScope&) = delete;
+  StreamScope& operator=(const StreamScope&) = delete;
+
+ private:
+  hipStream_t prev_;
+};
 void* dmalloc(size_t bytes);        // throws Error(FS2_ERR_MEMORY) on OOM
 void dfree(void* p);
 void sync();
@@ -69,6 +82,8 @@ void logf(int level, const char* fmt, ...);
 // fs2_profile_enable(1) was called.
 int prof_begin(const char* name, double bytes);
 void prof_end(int token);
+bool profiling_enabled();
+void set_profiling(bool on);
 
 template <typename K, typename... Args>
 inline void launch(const char* name, double bytes, K kernel, dim3 grid, dim3 block,
@@ -90,10 +105,12 @@ inline int grid_rpitch(int n) { return 2 * grid_cpitch(n); }
 inline size_t grid_floats(int n) { return (size_t)n * n * grid_rpitch(n); }
 
 // ---- FFT (fft.hip) ------------------------------------------------------------
-// Complex values are in natural [kx][ky][kz] order (kz = 0..n/2).
+// Complex values are in natural [kx][ky][kz] order (kz = 0..n/2).  nb > 1
+// transforms nb meshes stored back to back (grid_floats(n) apart) in one
+// launch per pass.
 bool fft_supported(int n);
-void fft_r2c_3d(float* grid, int n);
-void fft_c2r_3d(float* grid, int n);
+void fft_r2c_3d(float* grid, int n, int nb = 1);
+void fft_c2r_3d(float* grid, int n, int nb = 1);
 // Green x gradient fused into the first inverse pass (x pencils):
 //   dst = IFFT_x( i f2(k) delta_k ),  f2 = f1 k_axis / |k|^2, Nyquist -> +n/2
 // then the y pass and the z c2r pass in place on dst.
@@ -102,7 +119,7 @@ void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f
 // (z and y passes), then one fused x pass that writes the three Green x
 // gradient spectra, then y and z c2r passes of each force mesh.  The density
 // mesh is left holding its y-transformed intermediate.
-void fft_force_meshes(float* mesh, float* const out[3], int n, float f1);
+void fft_force_meshes(float* mesh, float* const out[3], int n, float f1, int nb = 1);
 
 // ---- PM kernels (pm_kernels.hip) -----------------------------------------------
 struct PosArgs {
@@ -129,25 +146,37 @@ struct PaintScratch {
   unsigned* amax = nullptr;      // [4] per-field max |value| of the adjoint paint
   size_t np = 0, nbins = 0;
 };
-size_t paint_tiled_bins(int n);
+// Tile extents and batching of the tiled kernels.  Zero extents pick the
+// default 8 x 16 x 32 (clamped to n); extents are powers of two dividing n.
+// nbox > 1: the particle planes hold nbox boxes of np / nbox particles each,
+// box-major, and the meshes are nbox grids of grid_floats(n) floats back to
+// back.
+struct TileOpts {
+  int tx = 0, ty = 0, tz = 0;
+  int nbox = 1;
+};
+size_t paint_tiled_bins(int n, const TileOpts& o = TileOpts());
+void tile_extents(int n, const TileOpts& o, int ext[3]);   // resolved tile extents
 void ensure_paint_scratch(PaintScratch& s, size_t np, size_t nbins);   // grows on demand
 void free_paint_scratch(PaintScratch& s);
 void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
-                 bool deterministic, const PaintScratch& s);
+                 bool deterministic, const PaintScratch& s, const TileOpts& o = TileOpts());
 // Readout of the three force meshes at the particles binned by the last
 // paint_tiled call (positions unchanged since); forces land at f[i + a*fstride].
 void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, int64_t fstride,
-                   int64_t np, int n, float boxsize, const PaintScratch& s);
+                   int64_t np, int n, float boxsize, const PaintScratch& s,
+                   const TileOpts& o = TileOpts());
 // Adjoint kernels over the bins of the last paint_tiled call (positions
 // unchanged since).  paint_values3_tiled: out_a(c) = sum_p W(x_p - c) v_a(p)
 // for the value planes v_a = vals[i + a*vstride], int64 fixed-point sums, every
 // cell written.  grad_readout_tiled: xbar_p += sum_c grad W(x_p - c)
 // [fac rhobar(c) + sum_a fbar_a(p) F_a(c)] (F may be null: rhobar term only).
 void paint_values3_tiled(const float* vals, int64_t vstride, int64_t np, float* const out[3],
-                         int n, float boxsize, const PaintScratch& s);
+                         int n, float boxsize, const PaintScratch& s,
+                         const TileOpts& o = TileOpts());
 void grad_readout_tiled(const float* rhobar, float fac, const float* const F[3],
                         const float* fbar, float* xbar, int64_t stride, int64_t np, int n,
-                        float boxsize, const PaintScratch& s);
+                        float boxsize, const PaintScratch& s, const TileOpts& o = TileOpts());
 void power_spectrum(const float* delta_k, int n, double boxsize, int nbins,
                     double dk_bin, double* k, double* P, double* nmodes);
 

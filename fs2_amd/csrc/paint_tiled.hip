@@ -73,6 +73,12 @@ struct TileGeom {
   int lx, ly, lz;      // log2 of the extents
   int ntx, nty, ntz;   // tiles per axis
   float dx_inv;
+  // batched launches: nbox boxes of npb particles each (particle i belongs
+  // to box i / npb), meshes mstride floats apart, bins_box = 8 * ntiles bins
+  // per box; workgroup b works on tile b % ntiles of box b / ntiles.
+  int ntiles, nbox;
+  unsigned npb, bins_box;
+  int64_t mstride;
 };
 
 __device__ __forceinline__ unsigned key_of(const TileGeom& g, int ix, int iy, int iz) {
@@ -128,6 +134,7 @@ __global__ void k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L
     const Cic1 cy = cic1(p[1], g.dx_inv, g.n);
     const Cic1 cz = cic1(p[2], g.dx_inv, g.n);
     key = key_of(g, cx.i0, cy.i0, cz.i0);
+    if (g.nbox > 1) key += ((unsigned)i / g.npb) * g.bins_box;
     keys[i] = key;
   }
   wave_aggregated_inc(counts, key, valid);
@@ -291,11 +298,15 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
   for (int c = threadIdx.x; c < tcells; c += blockDim.x) acc[c] = (ACC)0;
   __syncthreads();
 
-  const int tile = blockIdx.x;
+  const int box = blockIdx.x / g.ntiles;
+  const int tile = blockIdx.x - box * g.ntiles;
+  offsets += (int64_t)box * g.bins_box;
+  const int64_t moff = (int64_t)box * g.mstride;
   const int Tz = tile % g.ntz;
   const int Ty = (tile / g.ntz) % g.nty;
   const int Tx = tile / (g.ntz * g.nty);
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
+  grid += moff;
 
   __shared__ unsigned offs[8][9];
   __shared__ unsigned rb[16], rpre[17];
@@ -366,12 +377,15 @@ k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offs
   extern __shared__ float hs[];
   const int hx = g.tx + 1, hy = g.ty + 1, hz = g.tz + 1;
   const int hcells = hx * hy * hz;
-  const int tile = blockIdx.x;
+  const int box = blockIdx.x / g.ntiles;
+  const int tile = blockIdx.x - box * g.ntiles;
+  offsets += (int64_t)box * g.bins_box;
+  const int64_t moff = (int64_t)box * g.mstride;
   const int Tz = tile % g.ntz;
   const int Ty = (tile / g.ntz) % g.nty;
   const int Tx = tile / (g.ntz * g.nty);
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
-  const float* ms[3] = {m0, m1, m2};
+  const float* ms[3] = {m0 + moff, m1 + moff, m2 + moff};
   for (int c = threadIdx.x; c < hcells; c += blockDim.x) {
     const int lz = c % hz;
     const int r = c / hz;
@@ -464,7 +478,10 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
   const int tcells = g.tx * g.ty * g.tz;
   for (int c = threadIdx.x; c < 3 * tcells; c += blockDim.x) acc[c] = 0ull;
 
-  const int tile = blockIdx.x;
+  const int box = blockIdx.x / g.ntiles;
+  const int tile = blockIdx.x - box * g.ntiles;
+  offsets += (int64_t)box * g.bins_box;
+  const int64_t moff = (int64_t)box * g.mstride;
   const int Tz = tile % g.ntz;
   const int Ty = (tile / g.ntz) % g.nty;
   const int Tx = tile / (g.ntz * g.nty);
@@ -518,7 +535,7 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
     }
   }
   __syncthreads();
-  float* outs[3] = {o0, o1, o2};
+  float* outs[3] = {o0 + moff, o1 + moff, o2 + moff};
   for (int c = threadIdx.x; c < tcells; c += blockDim.x) {
     const int lz = c & (g.tz - 1);
     const int ly = (c >> g.lz) & (g.ty - 1);
@@ -546,12 +563,16 @@ k_tile_grad_readout(const float4* __restrict__ rec, const unsigned* __restrict__
   constexpr int NM = FORCES ? 4 : 1;
   const int hx = g.tx + 1, hy = g.ty + 1, hz = g.tz + 1;
   const int hcells = hx * hy * hz;
-  const int tile = blockIdx.x;
+  const int box = blockIdx.x / g.ntiles;
+  const int tile = blockIdx.x - box * g.ntiles;
+  offsets += (int64_t)box * g.bins_box;
+  const int64_t moff = (int64_t)box * g.mstride;
   const int Tz = tile % g.ntz;
   const int Ty = (tile / g.ntz) % g.nty;
   const int Tx = tile / (g.ntz * g.nty);
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
-  const float* ms[4] = {rhobar, m0, m1, m2};
+  const float* ms[4] = {rhobar + moff, FORCES ? m0 + moff : nullptr, FORCES ? m1 + moff : nullptr,
+                        FORCES ? m2 + moff : nullptr};
   for (int c = threadIdx.x; c < hcells; c += blockDim.x) {
     const int lz = c % hz;
     const int r = c / hz;
@@ -603,41 +624,73 @@ k_tile_grad_readout(const float4* __restrict__ rec, const unsigned* __restrict__
   }
 }
 
-TileGeom make_geom(int n, float boxsize) {
+TileGeom make_geom(int n, float boxsize, const TileOpts& o, int64_t np = 0) {
   TileGeom g;
   g.n = n;
   g.ncz = grid_rpitch(n);
-  // 8 x 16 x 32 cells: 16 KB of FP32 LDS, 128-byte rows along z
-  g.tx = n % 8 == 0 ? 8 : n;
-  g.ty = n % 16 == 0 ? 16 : n;
-  g.tz = n % 32 == 0 ? 32 : (n % 16 == 0 ? 16 : n);
+  // default 8 x 16 x 32 cells: 16 KB of FP32 LDS, 128-byte rows along z
+  g.tx = o.tx ? o.tx : (n % 8 == 0 ? 8 : n);
+  g.ty = o.ty ? o.ty : (n % 16 == 0 ? 16 : n);
+  g.tz = o.tz ? o.tz : (n % 32 == 0 ? 32 : (n % 16 == 0 ? 16 : n));
+  if ((g.tx & (g.tx - 1)) || (g.ty & (g.ty - 1)) || (g.tz & (g.tz - 1)) || g.tx < 1 ||
+      g.ty < 1 || g.tz < 1)
+    fail(FS2_ERR_VALUE, "paint tiles need power-of-two extents (n = %d, tile %dx%dx%d)", n, g.tx,
+         g.ty, g.tz);
+  if (n % g.tx || n % g.ty || n % g.tz)
+    fail(FS2_ERR_VALUE, "tile %dx%dx%d does not divide the %d^3 mesh", g.tx, g.ty, g.tz, n);
   g.lx = __builtin_ctz((unsigned)g.tx);
   g.ly = __builtin_ctz((unsigned)g.ty);
   g.lz = __builtin_ctz((unsigned)g.tz);
-  if ((g.tx & (g.tx - 1)) || (g.ty & (g.ty - 1)) || (g.tz & (g.tz - 1)))
-    fail(FS2_ERR_VALUE, "paint tiles need power-of-two extents (n = %d)", n);
   g.ntx = n / g.tx;
   g.nty = n / g.ty;
   g.ntz = n / g.tz;
   g.dx_inv = (float)n / boxsize;   // pm.cpp:77
+  g.ntiles = g.ntx * g.nty * g.ntz;
+  g.bins_box = (unsigned)g.ntiles * 8u;
+  g.nbox = o.nbox < 1 ? 1 : o.nbox;
+  if (np % g.nbox) fail(FS2_ERR_VALUE, "%lld particles do not split into %d boxes",
+                        (long long)np, g.nbox);
+  g.npb = (unsigned)(np / g.nbox);
+  g.mstride = (int64_t)grid_floats(n);
   return g;
+}
+
+constexpr size_t kMaxLds = 160 * 1024;
+
+void check_lds(size_t bytes, const char* what) {
+  if (bytes > kMaxLds)
+    fail(FS2_ERR_VALUE, "%s: tile needs %zu bytes of LDS (limit %zu)", what, bytes, kMaxLds);
+}
+
+void check_scratch(const TileGeom& g, int64_t np, const PaintScratch& s) {
+  const int64_t nbins = (int64_t)g.nbox * g.bins_box;
+  if ((size_t)nbins > s.nbins || (size_t)np > s.np)
+    fail(FS2_ERR_RUNTIME, "paint scratch too small (%lld bins, %lld particles)",
+         (long long)nbins, (long long)np);
 }
 
 }  // namespace
 
-size_t paint_tiled_bins(int n) {
-  const TileGeom g = make_geom(n, 1.0f);
-  return (size_t)g.ntx * g.nty * g.ntz * 8;
+void tile_extents(int n, const TileOpts& o, int ext[3]) {
+  const TileGeom g = make_geom(n, 1.0f, o);
+  ext[0] = g.tx;
+  ext[1] = g.ty;
+  ext[2] = g.tz;
+}
+
+size_t paint_tiled_bins(int n, const TileOpts& o) {
+  const TileGeom g = make_geom(n, 1.0f, o);
+  return (size_t)g.nbox * g.bins_box;
 }
 
 void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
-                 bool deterministic, const PaintScratch& s) {
-  const TileGeom g = make_geom(n, boxsize);
-  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
-  const int64_t nbins = ntiles * 8;
-  if ((size_t)nbins > s.nbins || (size_t)p.np > s.np)
-    fail(FS2_ERR_RUNTIME, "paint scratch too small (%lld bins, %lld particles)",
-         (long long)nbins, (long long)p.np);
+                 bool deterministic, const PaintScratch& s, const TileOpts& o) {
+  const TileGeom g = make_geom(n, boxsize, o, p.np);
+  check_scratch(g, p.np, s);
+  const int64_t nbins = (int64_t)g.nbox * g.bins_box;
+  const int64_t nblk = (int64_t)g.nbox * g.ntiles;
+  const int tcells = g.tx * g.ty * g.tz;
+  check_lds(tcells * (deterministic ? sizeof(unsigned long long) : sizeof(float)), "paint");
   launch("paint_clear_bins", (double)nbins * 4.0, k_fill_u32, dim3(blocks_for(nbins)),
          dim3(kBlock), 0, s.counts, nbins, 0u);
   if (p.np > 0) {
@@ -658,61 +711,65 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
     launch("paint_scatter", (double)p.np * 28.0, k_scatter, dim3(blocks_for(p.np)), dim3(kBlock),
            0, (const float*)p.x, p.stride, p.np, (const unsigned*)s.keys, s.cursor, s.rec);
   }
-  const int tcells = g.tx * g.ty * g.tz;
-  const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0;
+  const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0 * g.nbox;
   if (deterministic) {
-    launch("paint_tile_det", bytes, k_tile_paint<unsigned long long>, dim3((unsigned)ntiles),
+    launch("paint_tile_det", bytes, k_tile_paint<unsigned long long>, dim3((unsigned)nblk),
            dim3(kBlock), tcells * sizeof(unsigned long long), (const float4*)s.rec,
            (const unsigned*)s.offsets, g, fac, grid);
   } else {
-    launch("paint_tile", bytes, k_tile_paint<float>, dim3((unsigned)ntiles), dim3(kBlock),
+    launch("paint_tile", bytes, k_tile_paint<float>, dim3((unsigned)nblk), dim3(kBlock),
            tcells * sizeof(float), (const float4*)s.rec, (const unsigned*)s.offsets,
            g, fac, grid);
   }
 }
 
 void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, int64_t fstride,
-                   int64_t np, int n, float boxsize, const PaintScratch& s) {
-  const TileGeom g = make_geom(n, boxsize);
-  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
+                   int64_t np, int n, float boxsize, const PaintScratch& s, const TileOpts& o) {
+  const TileGeom g = make_geom(n, boxsize, o, np);
+  check_scratch(g, np, s);
+  const int64_t nblk = (int64_t)g.nbox * g.ntiles;
   const size_t lds = (size_t)3 * (g.tx + 1) * (g.ty + 1) * (g.tz + 1) * sizeof(float);
-  const double bytes = (double)np * 24.0 + 3.0 * 4.0 * (double)n * n * n;
-  launch("readout_tile", bytes, k_tile_readout, dim3((unsigned)ntiles), dim3(512), lds,
+  check_lds(lds, "readout");
+  const double bytes = (double)np * 24.0 + 3.0 * 4.0 * (double)n * n * n * g.nbox;
+  launch("readout_tile", bytes, k_tile_readout, dim3((unsigned)nblk), dim3(512), lds,
          (const float4*)s.rec, (const unsigned*)s.offsets, g, g0, g1, g2, f, fstride);
 }
 
 void paint_values3_tiled(const float* vals, int64_t vstride, int64_t np, float* const out[3],
-                         int n, float boxsize, const PaintScratch& s) {
-  const TileGeom g = make_geom(n, boxsize);
-  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
-  if ((size_t)(ntiles * 8) > s.nbins || (size_t)np > s.np)
-    fail(FS2_ERR_RUNTIME, "paint scratch does not hold bins for this mesh / particle count");
+                         int n, float boxsize, const PaintScratch& s, const TileOpts& o) {
+  const TileGeom g = make_geom(n, boxsize, o, np);
+  check_scratch(g, np, s);
+  const int64_t nblk = (int64_t)g.nbox * g.ntiles;
+  const int tcells = g.tx * g.ty * g.tz;
+  check_lds(3 * tcells * sizeof(unsigned long long), "adjoint paint");
   launch("adj_absmax_clear", 12.0, k_fill_u32, dim3(1), dim3(kBlock), 0, s.amax, (int64_t)3, 0u);
   if (np > 0)
     launch("adj_absmax3", (double)np * 12.0, k_absmax3,
            dim3(std::min<unsigned>(blocks_for(np), 1024u)), dim3(kBlock), 0, vals, vstride, np,
            s.amax);
-  const int tcells = g.tx * g.ty * g.tz;
-  launch("adj_paint_tile3", (double)np * 28.0 + 12.0 * (double)n * n * n, k_tile_paint_values3,
-         dim3((unsigned)ntiles), dim3(512), 3 * tcells * sizeof(unsigned long long),
-         (const float4*)s.rec, (const unsigned*)s.offsets, g, vals, vstride, (float)np,
-         (const unsigned*)s.amax, out[0], out[1], out[2]);
+  launch("adj_paint_tile3", (double)np * 28.0 + 12.0 * (double)n * n * n * g.nbox,
+         k_tile_paint_values3, dim3((unsigned)nblk), dim3(512),
+         3 * tcells * sizeof(unsigned long long), (const float4*)s.rec,
+         (const unsigned*)s.offsets, g, vals, vstride, (float)np, (const unsigned*)s.amax, out[0],
+         out[1], out[2]);
 }
 
 void grad_readout_tiled(const float* rhobar, float fac, const float* const F[3],
                         const float* fbar, float* xbar, int64_t stride, int64_t np, int n,
-                        float boxsize, const PaintScratch& s) {
-  const TileGeom g = make_geom(n, boxsize);
-  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
+                        float boxsize, const PaintScratch& s, const TileOpts& o) {
+  const TileGeom g = make_geom(n, boxsize, o, np);
+  check_scratch(g, np, s);
+  const int64_t nblk = (int64_t)g.nbox * g.ntiles;
   const size_t hcells = (size_t)(g.tx + 1) * (g.ty + 1) * (g.tz + 1);
+  check_lds((F ? 4 : 1) * hcells * sizeof(float), "gradient readout");
   if (F) {
-    launch("adj_grad_tile4", (double)np * 40.0 + 16.0 * (double)n * n * n,
-           k_tile_grad_readout<true>, dim3((unsigned)ntiles), dim3(512), 4 * hcells * sizeof(float),
+    launch("adj_grad_tile4", (double)np * 40.0 + 16.0 * (double)n * n * n * g.nbox,
+           k_tile_grad_readout<true>, dim3((unsigned)nblk), dim3(512), 4 * hcells * sizeof(float),
            (const float4*)s.rec, (const unsigned*)s.offsets, g, rhobar, fac, F[0], F[1], F[2],
            fbar, xbar, stride);
   } else {
-    launch("adj_grad_tile1", (double)np * 28.0 + 4.0 * (double)n * n * n,
-           k_tile_grad_readout<false>, dim3((unsigned)ntiles), dim3(512), hcells * sizeof(float),
+    launch("adj_grad_tile1", (double)np * 28.0 + 4.0 * (double)n * n * n * g.nbox,
+           k_tile_grad_readout<false>, dim3((unsigned)nblk), dim3(512), hcells * sizeof(float),
            (const float4*)s.rec, (const unsigned*)s.offsets, g, rhobar, fac, nullptr, nullptr,
            nullptr, nullptr, xbar, stride);
   }

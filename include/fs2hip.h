@@ -71,6 +71,7 @@ typedef struct fs2_pm fs2_pm;
 typedef struct fs2_grid fs2_grid;
 typedef struct fs2_power fs2_power;
 typedef struct fs2_diff fs2_diff;
+typedef struct fs2_batch fs2_batch;
 
 /* ---- library ---------------------------------------------------------- */
 const char* fs2_last_error(void);
@@ -267,6 +268,40 @@ int fs2_diff_set_paint_variant(fs2_diff* d, int variant);
 int fs2_diff_forward(fs2_diff* d, const float* noise, float* delta);
 /* delta_bar: d loss / d delta (nc_pm^3); noise_bar: d loss / d noise (nc_lpt^3). */
 int fs2_diff_backward(fs2_diff* d, const float* delta_bar, float* noise_bar);
+
+/* ---- batched ensemble ------------------------------------------------------- */
+/* No reference counterpart: the reference runs one box per process
+ * (script/cola.py:28-48).  A batch holds nclass classes; class k has
+ * nbox[k] independent boxes of nc_lpt[k]^3 particles on nc_pm[k]^3 meshes
+ * of side boxsize[k].  Every kernel of a step covers a whole class; each
+ * class runs on its own stream, and the nstep-step run of all classes is
+ * captured once into a hipGraph and replayed with one launch.  Per box the
+ * results are bit-identical to fs2_lpt + fs2_pm_* + fs2_cola_* on that box. */
+int fs2_batch_create(int nclass, const int* nc_lpt, const int* nc_pm, const int* nbox,
+                     const double* boxsize, fs2_batch** out);
+void fs2_batch_destroy(fs2_batch* b);
+int fs2_batch_info(const fs2_batch* b, int* nclass, uint64_t* np_total);
+/* 2LPT ('cola' kind, v = 0) at scale factor a for every box; seeds holds one
+ * seed per box, class-major.  Kept as the state fs2_batch_reset restores. */
+int fs2_batch_init_lpt(fs2_batch* b, const fs2_power* ps, double a, const uint64_t* seeds);
+/* Paint / readout tile extents of a class (powers of two dividing nc_pm). */
+int fs2_batch_set_tile(fs2_batch* b, int cls, int tx, int ty, int tz);
+int fs2_batch_get_tile(const fs2_batch* b, int cls, int* tx, int* ty, int* tz);
+/* Times every candidate tile shape of every class on its own data and keeps
+ * the fastest; best_ms (nclass entries, may be null) receives the time of
+ * one bin + paint + readout pass with the chosen shape.  Leaves the state
+ * at the initial conditions. */
+int fs2_batch_autotune(fs2_batch* b, int reps, double* best_ms);
+/* Steps a0 -> a1 in nstep COLA steps (schedule of script/cola.py:37-44). */
+int fs2_batch_set_schedule(fs2_batch* b, double a0, double a1, int nstep);
+int fs2_batch_capture(fs2_batch* b);
+/* Runs the schedule from the current state: use_graph = 1 replays the
+ * captured graph (capturing first if needed), 0 launches eagerly.
+ * Asynchronous on the library stream. */
+int fs2_batch_run(fs2_batch* b, int use_graph);
+int fs2_batch_reset(fs2_batch* b);
+/* One box's field (FS2_FIELD_*) as (nc_lpt^3, 3) FP32 rows on the host. */
+int fs2_batch_get_field(fs2_batch* b, int cls, int box, int field, float* out);
 
 /* ---- kernel timing --------------------------------------------------------- */
 /* No reference counterpart (the reference has no timers).  When enabled,
