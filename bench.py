@@ -185,39 +185,35 @@ def cpu_baseline(n_p, nc, box, sub, workers):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--np", type=int, default=512, help="particles per dimension")
-    ap.add_argument("--nc", type=int, default=1024, help="PM mesh per dimension")
-    ap.add_argument("--box", type=float, default=1024.0)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
-    ap.add_argument("--cpu-sub", type=int, default=8)
-    ap.add_argument("--adjoint", default="on", choices=["on", "off"],
-                    help="also time the differentiable 128^3/256^3 run (adjoint/forward ratio)")
-    ap.add_argument("--batch", default="on", choices=["on", "off"],
-                    help="also time the 96-box batched ensemble in one hipGraph")
-    args = ap.parse_args()
+PROFILE_TABLE = os.path.join(ROOT, "profiles", "cola_512p_1024mesh_kernels.md")
 
-    if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "
-              f"(the multi-GPU slab decomposition is design-only).", file=sys.stderr)
-        sys.exit(2)
 
-    import fs2_amd.fs as fs
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (read + write), from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary committed under profiles/ for this
+    workload, or None.  Rows: | `rocprof name` | short | calls | avg us |
+    read GB | write GB | total GB |."""
+    try:
+        with open(PROFILE_TABLE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 7 and cells[1] == kernel:
+                    return float(cells[6]) * 1e9
+    except (OSError, ValueError):
+        return None
+    return None
+
+
+def main_leg(args, fs, ps):
+    """The headline: COLA particle-steps/s of one box on one GPU."""
     c = fs._fs
-
-    fs.cosmology.init(0.31)
-    ps = fs.PowerSpectrum.eisenstein_hu(0.31, 0.049, 0.67, 0.96, 0.81)
     print(f"[bench] 2LPT {args.np}^3 box {args.box}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     p = fs.lpt.init(args.np, args.box, 0.1, ps, args.seed, "cola")
     c._synchronize()
     t_ic = time.perf_counter() - t0
     fs.pm.init(args.nc, args.nc / args.np, args.box)
+    fs.pm.set_paint_variant("sorted")
     snap = fs.Particles(args.np**3, args.box)
     snap.copy_from(p)
 
@@ -229,6 +225,7 @@ def main():
     c._synchronize()
     p.copy_from(snap)
     c._synchronize()
+    del snap
 
     c._profile_reset()
     c._profile_enable(True)
@@ -254,9 +251,9 @@ def main():
         c._synchronize()
         paint_ms[variant] = round((time.perf_counter() - t1) / 3 * 1e3, 3)
     fs.pm.set_paint_variant("sorted")
+    del p
 
     npart = args.np**3
-    value = npart * args.steps / T
     kernels = {}
     for name, launches, total_ms, bytes_per in prof:
         if launches == 0:
@@ -271,44 +268,81 @@ def main():
     ng = args.nc**3
     nk = args.nc**2 * (args.nc // 2 + 1)
     step_bytes = 156.0 * npart + 32.0 * ng + 32.0 * nk
-    out = {
-        "metric": "COLA particle-steps/s, 512^3 p/1024^3 mesh, 1 GPU",
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+    headline = (args.np, args.nc, args.box) == (512, 1024, 1024.0)
+    return {
+        "value": npart * args.steps / T,
         "ms_per_step": T / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (2LPT from seeded white noise, EH98 no-wiggle P(k))",
         "config": {"workload": f"COLA {args.np}^3 particles, {args.nc}^3 mesh, box "
                                f"{args.box:g} Mpc/h, {args.steps} steps a=0.1->1",
                    "np": args.np, "nc": args.nc, "box": args.box,
                    "paint": "sorted (deterministic, default)", "ic_seconds": round(t_ic, 3)},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "algorithmic_bytes": dom[3], "avg_ms": round(dom_avg_s * 1e3, 4),
+                     "traffic": measured_traffic(dom[0]) if headline else None},
         "step_roofline": {"algorithmic_bytes": step_bytes,
                           "achieved_GBps": round(step_bytes / (T / args.steps) / 1e9, 1),
                           "frac": round(step_bytes / (T / args.steps) / 1e9 / PEAK_HBM_GBS, 4)},
         "kernels": kernels,
         "paint_variants_ms_final_state": paint_ms,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--np", type=int, default=512, help="particles per dimension")
+    ap.add_argument("--nc", type=int, default=1024, help="PM mesh per dimension")
+    ap.add_argument("--box", type=float, default=1024.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--cpu-sub", type=int, default=8)
+    ap.add_argument("--legs", default="main,adjoint,batch",
+                    help="comma list of main (headline COLA box), adjoint (differentiable "
+                         "128^3/256^3 run), batch (96-box hipGraph ensemble)")
+    args = ap.parse_args()
+    legs = set(x.strip() for x in args.legs.split(",") if x.strip())
+
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "
+              f"(the multi-GPU slab decomposition is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    import fs2_amd.fs as fs
+    fs.cosmology.init(0.31)
+    ps = fs.PowerSpectrum.eisenstein_hu(0.31, 0.049, 0.67, 0.96, 0.81)
+
+    out = {
+        "metric": "COLA particle-steps/s, 512^3 p/1024^3 mesh, 1 GPU",
+        "value": None,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (2LPT from seeded white noise, EH98 no-wiggle P(k))",
+        "config": None,
+        "roofline": None,
         "adjoint": None,
         "batch": None,
         "cpu_baseline": None,
     }
-    if args.adjoint == "on":
-        del snap
+    if "main" in legs:
+        out.update(main_leg(args, fs, ps))
+    if "adjoint" in legs:
         print("[bench] adjoint 128^3/256^3", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(fs, ps)
-    if args.batch == "on":
+    if "batch" in legs:
         print("[bench] batched ensemble (96 boxes, hipGraph)", file=sys.stderr, flush=True)
         out["batch"] = batch_leg(fs, ps)
-    if args.cpu_baseline == "on":
-        del p
+    if args.cpu_baseline == "on" and "main" in legs:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         try:
             workers = min(16, len(os.sched_getaffinity(0)))

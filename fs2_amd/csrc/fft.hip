@@ -181,21 +181,75 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
   __syncthreads();
 }
 
+// Register-resident variant.  Thread t holds elements t + i T (i < N/T) of
+// the pencil in x[i]; on return x[i] holds output element t + i T.  The
+// first Stockham stage (span 1) reads exactly x[b + r B] (B = N/(R T)) and
+// the last stage (span N/R) writes exactly x[b + r B], so only the stores of
+// the first stage and the inner stages go through LDS: two LDS passes fewer
+// and three barriers instead of 2S + 1.  Starts with a barrier, so the LDS
+// pencil may still be in use by the caller on entry.
+template <int N, int T, class L, int R>
+__device__ __forceinline__ void stage_first(const float2* x, float2* s, int t) {
+  constexpr int B = N / (R * T);
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = x[b + r * B];
+    DFT<R>::run(v);
+    const int j = t + b * T;
+#pragma unroll
+    for (int r = 0; r < R; ++r) s[L::at(j * R + r)] = v[r];
+  }
+  __syncthreads();
+}
+
+template <int N, int T, class L, int Ns, int R>
+__device__ __forceinline__ void stage_last(float2* x, const float2* s, int t,
+                                           const float2* __restrict__ tw) {
+  static_assert(Ns * R == N, "plan: last stage");
+  constexpr int B = N / (R * T);
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + b * T;   // j < Ns, so the output base is j itself
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[L::at(j + r * (N / R))];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * j * (N / (Ns * R))]);
+    DFT<R>::run(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[b + r * B] = v[r];
+  }
+}
+
 template <int N, int T, class L, int Ns, int R, int... Rest>
-__device__ __forceinline__ void stockham_stages(float2* s, int t, const float2* __restrict__ tw) {
-  stockham_stage<N, T, L, Ns, R>(s, t, tw);
-  if constexpr (sizeof...(Rest) > 0) stockham_stages<N, T, L, Ns * R, Rest...>(s, t, tw);
+__device__ __forceinline__ void regs_stages(float2* x, float2* s, int t,
+                                            const float2* __restrict__ tw) {
+  if constexpr (sizeof...(Rest) == 0) {
+    stage_last<N, T, L, Ns, R>(x, s, t, tw);
+  } else {
+    stockham_stage<N, T, L, Ns, R>(s, t, tw);
+    regs_stages<N, T, L, Ns * R, Rest...>(x, s, t, tw);
+  }
 }
 
-template <int N, int T, class L, int... Rs>
-__device__ __forceinline__ void run_plan(float2* s, int t, const float2* __restrict__ tw, Radices<Rs...>) {
-  stockham_stages<N, T, L, 1, Rs...>(s, t, tw);
+template <int N, int T, class L, int R1, int... Rest>
+__device__ __forceinline__ void run_plan_regs(float2* x, float2* s, int t,
+                                              const float2* __restrict__ tw, Radices<R1, Rest...>) {
+  if constexpr (sizeof...(Rest) == 0) {
+    static_assert(T == 1 && R1 == N, "single-stage plans run in one thread");
+    DFT<N>::run(x);
+  } else {
+    __syncthreads();
+    stage_first<N, T, L, R1>(x, s, t);
+    regs_stages<N, T, L, R1, Rest...>(x, s, t, tw);
+  }
 }
 
-// Forward FFT of length N of the pencil s[L::at(j)], j < N, by the T threads t.
 template <int N, class L>
-__device__ __forceinline__ void fft_lds(float2* s, int t, const float2* __restrict__ tw) {
-  run_plan<N, Plan<N>::T, L>(s, t, tw, typename Plan<N>::R{});
+__device__ __forceinline__ void fft_regs(float2* x, float2* s, int t, const float2* __restrict__ tw) {
+  run_plan_regs<N, Plan<N>::T, L>(x, s, t, tw, typename Plan<N>::R{});
 }
 
 // Copies the N-entry twiddle table into LDS (read there by every stage; a
@@ -203,14 +257,6 @@ __device__ __forceinline__ void fft_lds(float2* s, int t, const float2* __restri
 template <int N>
 __device__ __forceinline__ void stage_twiddles(float2* stw, const float2* __restrict__ tw) {
   for (int i = threadIdx.x; i < N; i += blockDim.x) stw[i] = tw[i];
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs; remap so that
-// consecutive logical tiles run on the same XCD and share its L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
-  const int per = nblocks / 8, rem = nblocks % 8;
-  const int xcd = bid % 8, local = bid / 8;
-  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
 // ---- Green x gradient multiplier (py/pm.cpp:460-504) ---------------------------------------
@@ -256,6 +302,7 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
   // 32-bit element offsets: grids up to 1024^3 stay below 2^31 float2
   const int base = o * (int)outer_stride + c;
   const int es = (int)elem_stride;
+  float2 x[N / T];
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
     const int j = t + i * T;
@@ -263,16 +310,14 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
     if (valid) v = src[base + j * es];
     if constexpr (GREEN) v = green_mul(v, j, o, c, g);
     if constexpr (INV) v.y = -v.y;
-    s[L::at(j) + p] = v;
+    x[i] = v;
   }
-  __syncthreads();
-  fft_lds<N, L>(s + p, t, stw);
+  fft_regs<N, L>(x, s + p, t, stw);
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
-    const int j = t + i * T;
-    float2 v = s[L::at(j) + p];
+    float2 v = x[i];
     if constexpr (INV) v.y = -v.y;
-    if (valid) dst[base + j * es] = v;
+    if (valid) dst[base + (t + i * T) * es] = v;
   }
 }
 
@@ -304,18 +349,13 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int 
   const bool valid = c < ncols;
   const int base = o * (int)outer_stride + c;
   const int es = (int)elem_stride;
+  float2 dk[N / T];
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
     const int j = t + i * T;
-    float2 v = make_float2(0.f, 0.f);
-    if (valid) v = src[base + j * es];
-    s[L::at(j) + p] = v;
+    dk[i] = valid ? src[base + j * es] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  fft_lds<N, L>(s + p, t, stw);
-  float2 dk[N / T];
-#pragma unroll
-  for (int i = 0; i < N / T; ++i) dk[i] = s[L::at(t + i * T) + p];
+  fft_regs<N, L>(dk, s + p, t, stw);
   float2* dsts[3] = {dst0, dst1, dst2};
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
@@ -325,23 +365,20 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int 
     // of them live, halving occupancy.
     int tl = t, pl = p;
     asm volatile("" : "+v"(tl), "+v"(pl));
-    __syncthreads();
+    float2 x[N / T];
 #pragma unroll
     for (int i = 0; i < N / T; ++i) {
-      const int j = tl + i * T;
-      float2 v = green_mul(dk[i], j, o, c, g);
+      float2 v = green_mul(dk[i], tl + i * T, o, c, g);
       v.y = -v.y;
-      s[L::at(j) + pl] = v;
+      x[i] = v;
     }
-    __syncthreads();
-    fft_lds<N, L>(s + pl, tl, stw);
+    fft_regs<N, L>(x, s + pl, tl, stw);
     float2* dst = dsts[a];
 #pragma unroll
     for (int i = 0; i < N / T; ++i) {
-      const int j = tl + i * T;
-      float2 v = s[L::at(j) + pl];
+      float2 v = x[i];
       v.y = -v.y;
-      if (valid) dst[base + j * es] = v;
+      if (valid) dst[base + (tl + i * T) * es] = v;
     }
   }
 }
@@ -364,13 +401,17 @@ k_fft_z_r2c(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
   const bool valid = row < nrows;
   float2* r = data + row * cpitch;
   float2* sp = s + p * LS;
+  float2 x[M / T];
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int m = t + i * T;
-    sp[L::at(m)] = valid ? r[m] : make_float2(0.f, 0.f);
+    x[i] = valid ? r[m] : make_float2(0.f, 0.f);
   }
+  fft_regs<M, L>(x, sp, t, twM);
+  __syncthreads();   // every thread is done reading the last stage's input
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = x[i];
   __syncthreads();
-  fft_lds<M, L>(sp, t, twM);
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
@@ -426,15 +467,11 @@ k_fft_z_c2r(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
     const float2 O = cmul(csub(A, Bc), cconj(twN[k]));
     z[i] = cconj(cadd(E, mul_posi(O)));
   }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = z[i];
-  __syncthreads();
-  fft_lds<M, L>(sp, t, twM);
+  fft_regs<M, L>(z, sp, t, twM);
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int m = t + i * T;
-    if (valid) r[m] = cconj(sp[L::at(m)]);
+    if (valid) r[m] = cconj(z[i]);
   }
 }
 
@@ -460,8 +497,8 @@ const float2* twiddles(int N) {
 template <int N> constexpr int pencil_cols() {
   return ((N + N / 16) * 16 * 8 <= 81920 && Plan<N>::T * 16 <= 1024) ? 16 : 8;
 }
-template <int M> constexpr int zrows() {
-  return Plan<M>::T >= 256 ? 1 : 256 / Plan<M>::T;
+template <int M> constexpr int zrows() {   // 128-thread blocks from M = 512 on (measured)
+  return Plan<M>::T >= 256 ? 1 : (M >= 512 ? 128 : 256) / Plan<M>::T;
 }
 
 template <int n>

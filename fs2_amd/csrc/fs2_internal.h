@@ -104,6 +104,15 @@ inline int grid_cpitch(int n) { return ((n / 2 + 1) + 15) & ~15; }
 inline int grid_rpitch(int n) { return 2 * grid_cpitch(n); }
 inline size_t grid_floats(int n) { return (size_t)n * n * grid_rpitch(n); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X; this
+// remaps the hardware block index so that consecutive logical blocks run on
+// the same XCD and share its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  const int per = nblocks / 8, rem = nblocks % 8;
+  const int xcd = bid % 8, local = bid / 8;
+  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
 // ---- FFT (fft.hip) ------------------------------------------------------------
 // Complex values are in natural [kx][ky][kz] order (kz = 0..n/2).  nb > 1
 // transforms nb meshes stored back to back (grid_floats(n) apart) in one

@@ -298,8 +298,10 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
   for (int c = threadIdx.x; c < tcells; c += blockDim.x) acc[c] = (ACC)0;
   __syncthreads();
 
-  const int box = blockIdx.x / g.ntiles;
-  const int tile = blockIdx.x - box * g.ntiles;
+  // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int box = bid / g.ntiles;
+  const int tile = bid - box * g.ntiles;
   offsets += (int64_t)box * g.bins_box;
   const int64_t moff = (int64_t)box * g.mstride;
   const int Tz = tile % g.ntz;
@@ -377,8 +379,10 @@ k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offs
   extern __shared__ float hs[];
   const int hx = g.tx + 1, hy = g.ty + 1, hz = g.tz + 1;
   const int hcells = hx * hy * hz;
-  const int box = blockIdx.x / g.ntiles;
-  const int tile = blockIdx.x - box * g.ntiles;
+  // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int box = bid / g.ntiles;
+  const int tile = bid - box * g.ntiles;
   offsets += (int64_t)box * g.bins_box;
   const int64_t moff = (int64_t)box * g.mstride;
   const int Tz = tile % g.ntz;
@@ -478,8 +482,10 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
   const int tcells = g.tx * g.ty * g.tz;
   for (int c = threadIdx.x; c < 3 * tcells; c += blockDim.x) acc[c] = 0ull;
 
-  const int box = blockIdx.x / g.ntiles;
-  const int tile = blockIdx.x - box * g.ntiles;
+  // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int box = bid / g.ntiles;
+  const int tile = bid - box * g.ntiles;
   offsets += (int64_t)box * g.bins_box;
   const int64_t moff = (int64_t)box * g.mstride;
   const int Tz = tile % g.ntz;
@@ -563,8 +569,10 @@ k_tile_grad_readout(const float4* __restrict__ rec, const unsigned* __restrict__
   constexpr int NM = FORCES ? 4 : 1;
   const int hx = g.tx + 1, hy = g.ty + 1, hz = g.tz + 1;
   const int hcells = hx * hy * hz;
-  const int box = blockIdx.x / g.ntiles;
-  const int tile = blockIdx.x - box * g.ntiles;
+  // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int box = bid / g.ntiles;
+  const int tile = bid - box * g.ntiles;
   offsets += (int64_t)box * g.bins_box;
   const int64_t moff = (int64_t)box * g.mstride;
   const int Tz = tile % g.ntz;
