@@ -58,8 +58,9 @@ namespace {
 
 // Tile shapes the autotuner tries (power-of-two extents; shapes that do not
 // divide the mesh or exceed the LDS budget are skipped).
-const int kShapes[][3] = {{8, 16, 32}, {8, 8, 32}, {16, 16, 16}, {8, 16, 16}, {4, 16, 32},
-                          {8, 8, 16},  {16, 8, 32}, {4, 8, 32},  {16, 16, 32}};
+const int kShapes[][3] = {{8, 8, 64},  {8, 16, 32}, {8, 8, 32}, {4, 8, 64},  {16, 16, 16},
+                          {8, 16, 16}, {4, 16, 32}, {8, 8, 16}, {16, 8, 32}, {4, 8, 32},
+                          {16, 16, 32}};
 
 bool shape_ok(int n, const int* t) {
   for (int a = 0; a < 3; ++a)

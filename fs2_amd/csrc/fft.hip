@@ -497,8 +497,8 @@ const float2* twiddles(int N) {
 template <int N> constexpr int pencil_cols() {
   return ((N + N / 16) * 16 * 8 <= 81920 && Plan<N>::T * 16 <= 1024) ? 16 : 8;
 }
-template <int M> constexpr int zrows() {   // 128-thread blocks from M = 512 on (measured)
-  return Plan<M>::T >= 256 ? 1 : (M >= 512 ? 128 : 256) / Plan<M>::T;
+template <int M> constexpr int zrows() {   // one-wave blocks from M = 512 on (measured)
+  return Plan<M>::T >= 256 ? 1 : (M >= 512 ? 64 : 256) / Plan<M>::T;
 }
 
 template <int n>

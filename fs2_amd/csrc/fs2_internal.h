@@ -156,7 +156,7 @@ struct PaintScratch {
   size_t np = 0, nbins = 0;
 };
 // Tile extents and batching of the tiled kernels.  Zero extents pick the
-// default 8 x 16 x 32 (clamped to n); extents are powers of two dividing n.
+// default 8 x 8 x 64 (clamped to n); extents are powers of two dividing n.
 // nbox > 1: the particle planes hold nbox boxes of np / nbox particles each,
 // box-major, and the meshes are nbox grids of grid_floats(n) floats back to
 // back.

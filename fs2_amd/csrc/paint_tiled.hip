@@ -35,6 +35,10 @@ namespace fs2 {
 namespace {
 
 constexpr int kBlock = 256;
+// Tile kernels: 512-thread paint and 1024-thread readout blocks keep 32
+// waves per CU with their LDS tiles (measured at 1024^3).
+constexpr int kTileBlock = 512;
+constexpr int kReadoutBlock = 1024;
 constexpr unsigned kNoKey = 0xffffffffu;
 constexpr double kFix = 4294967296.0;   // 2^32 fixed-point scale
 
@@ -289,7 +293,7 @@ __device__ __forceinline__ void tile_source_ranges(const unsigned* __restrict__ 
 // K4: one workgroup per tile.  ACC is float (LDS float atomics) or
 // unsigned long long (int64 fixed point, deterministic).
 template <typename ACC>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kTileBlock)
 k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
              TileGeom g, float fac, float* __restrict__ grid) {
   extern __shared__ unsigned char smem[];
@@ -372,7 +376,7 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
 // particles (every particle is in exactly one tile's bins) and writes the
 // forces back to the particles' id-order slots.  Same FP32 expression order
 // as the gather kernel.
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(kReadoutBlock)
 k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offsets, TileGeom g,
                const float* __restrict__ m0, const float* __restrict__ m1,
                const float* __restrict__ m2, float* __restrict__ f, int64_t fstride) {
@@ -636,10 +640,12 @@ TileGeom make_geom(int n, float boxsize, const TileOpts& o, int64_t np = 0) {
   TileGeom g;
   g.n = n;
   g.ncz = grid_rpitch(n);
-  // default 8 x 16 x 32 cells: 16 KB of FP32 LDS, 128-byte rows along z
+  // default 8 x 8 x 64 cells (measured fastest at 1024^3 for bin + scatter +
+  // paint + readout): 256-byte rows along z, a 63 KB readout halo tile
   g.tx = o.tx ? o.tx : (n % 8 == 0 ? 8 : n);
-  g.ty = o.ty ? o.ty : (n % 16 == 0 ? 16 : n);
-  g.tz = o.tz ? o.tz : (n % 32 == 0 ? 32 : (n % 16 == 0 ? 16 : n));
+  g.ty = o.ty ? o.ty : (n % 8 == 0 ? 8 : n);
+  g.tz = o.tz ? o.tz
+              : (n % 64 == 0 ? 64 : n % 32 == 0 ? 32 : n % 16 == 0 ? 16 : n);
   if ((g.tx & (g.tx - 1)) || (g.ty & (g.ty - 1)) || (g.tz & (g.tz - 1)) || g.tx < 1 ||
       g.ty < 1 || g.tz < 1)
     fail(FS2_ERR_VALUE, "paint tiles need power-of-two extents (n = %d, tile %dx%dx%d)", n, g.tx,
@@ -722,10 +728,10 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
   const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0 * g.nbox;
   if (deterministic) {
     launch("paint_tile_det", bytes, k_tile_paint<unsigned long long>, dim3((unsigned)nblk),
-           dim3(kBlock), tcells * sizeof(unsigned long long), (const float4*)s.rec,
+           dim3(kTileBlock), tcells * sizeof(unsigned long long), (const float4*)s.rec,
            (const unsigned*)s.offsets, g, fac, grid);
   } else {
-    launch("paint_tile", bytes, k_tile_paint<float>, dim3((unsigned)nblk), dim3(kBlock),
+    launch("paint_tile", bytes, k_tile_paint<float>, dim3((unsigned)nblk), dim3(kTileBlock),
            tcells * sizeof(float), (const float4*)s.rec, (const unsigned*)s.offsets,
            g, fac, grid);
   }
@@ -739,7 +745,7 @@ void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, 
   const size_t lds = (size_t)3 * (g.tx + 1) * (g.ty + 1) * (g.tz + 1) * sizeof(float);
   check_lds(lds, "readout");
   const double bytes = (double)np * 24.0 + 3.0 * 4.0 * (double)n * n * n * g.nbox;
-  launch("readout_tile", bytes, k_tile_readout, dim3((unsigned)nblk), dim3(512), lds,
+  launch("readout_tile", bytes, k_tile_readout, dim3((unsigned)nblk), dim3(kReadoutBlock), lds,
          (const float4*)s.rec, (const unsigned*)s.offsets, g, g0, g1, g2, f, fstride);
 }
 
