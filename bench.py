@@ -204,6 +204,27 @@ def measured_traffic(kernel):
     return None
 
 
+def copy_probe(gib=4):
+    """Device-to-device copy of `gib` GiB (torch's copy kernel), timed with
+    events: the practical ceiling of a 1-read + 1-write streaming pass."""
+    import torch
+    n = gib * (1 << 30) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return {"bytes": 2 * n * 4, "ms": round(ms, 4), "GBps": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
+
+
 def main_leg(args, fs, ps):
     """The headline: COLA particle-steps/s of one box on one GPU."""
     c = fs._fs
@@ -330,12 +351,16 @@ def main():
         "data": "synthetic (2LPT from seeded white noise, EH98 no-wiggle P(k))",
         "config": None,
         "roofline": None,
+        "copy_peak": None,
         "adjoint": None,
         "batch": None,
         "cpu_baseline": None,
     }
     if "main" in legs:
         out.update(main_leg(args, fs, ps))
+        out["copy_peak"] = copy_probe()
+        out["roofline"]["frac_of_copy_peak"] = round(
+            out["roofline"]["achieved"] / out["copy_peak"]["GBps"], 4)
     if "adjoint" in legs:
         print("[bench] adjoint 128^3/256^3", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(fs, ps)

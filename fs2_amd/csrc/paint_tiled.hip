@@ -94,23 +94,48 @@ __device__ __forceinline__ unsigned key_of(const TileGeom& g, int ix, int iy, in
   return tile * 8u + cls;
 }
 
-// Wavefront-aggregated atomicAdd of 1 per lane to ctr[key]; returns the
-// slot of this lane (old value + rank among lanes with the same key).
-__device__ __forceinline__ unsigned wave_aggregated_inc(unsigned* ctr, unsigned key, bool valid) {
-  const int lane = __lane_id();
+// Lanes of a wavefront grouped by equal key: on return *leader is the
+// lowest lane holding this lane's key and *same the mask of those lanes.
+// One ballot round per distinct key; no memory traffic.
+__device__ __forceinline__ void group_by_key(unsigned key, bool valid, int* leader,
+                                             uint64_t* same) {
   uint64_t active = __ballot(valid);
-  unsigned slot = 0;
+  const int lane = __lane_id();
+  *leader = lane;
+  *same = 0;
   while (active) {
-    const int leader = __ffsll((unsigned long long)active) - 1;
-    const unsigned lk = __shfl(key, leader);
-    const uint64_t same = __ballot(valid && key == lk) & active;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(ctr + lk, (unsigned)__popcll(same));
-    base = __shfl(base, leader);
-    if ((same >> lane) & 1ull) slot = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
-    active &= ~same;
+    const int l = __ffsll((unsigned long long)active) - 1;
+    const unsigned lk = __shfl(key, l);
+    const uint64_t m = __ballot(valid && key == lk) & active;
+    if ((m >> lane) & 1ull) {
+      *leader = l;
+      *same = m;
+    }
+    active &= ~m;
   }
-  return slot;
+}
+
+// Wavefront-aggregated atomicAdd of 1 per lane to ctr[key]; returns the
+// slot of this lane (old value + rank among lanes with the same key).  The
+// group leaders issue their atomics together, so a wavefront waits for one
+// atomic round trip however many distinct keys it holds.
+__device__ __forceinline__ unsigned wave_aggregated_inc(unsigned* ctr, unsigned key, bool valid) {
+  int leader;
+  uint64_t same;
+  group_by_key(key, valid, &leader, &same);
+  const int lane = __lane_id();
+  unsigned base = 0;
+  if (valid && lane == leader) base = atomicAdd(ctr + key, (unsigned)__popcll(same));
+  base = __shfl(base, leader);
+  return base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+}
+
+// Counting only: one atomic per distinct key and wavefront, result unused.
+__device__ __forceinline__ void wave_aggregated_count(unsigned* ctr, unsigned key, bool valid) {
+  int leader;
+  uint64_t same;
+  group_by_key(key, valid, &leader, &same);
+  if (valid && __lane_id() == leader) atomicAdd(ctr + key, (unsigned)__popcll(same));
 }
 
 __global__ void k_fill_u32(unsigned* __restrict__ a, int64_t n, unsigned v) {
@@ -141,7 +166,7 @@ __global__ void k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L
     if (g.nbox > 1) key += ((unsigned)i / g.npb) * g.bins_box;
     keys[i] = key;
   }
-  wave_aggregated_inc(counts, key, valid);
+  wave_aggregated_count(counts, key, valid);
 }
 
 // Exclusive scan of the bin counts in three launches: per-block scans of
