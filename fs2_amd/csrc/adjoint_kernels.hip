@@ -132,12 +132,12 @@ __global__ void k_green_sum_adj(float2* a0, const float2* __restrict__ a1,
     return;
   }
   const float k[3] = {(float)kx, (float)ky, (float)iz};
-  const float inv = f1 / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  const float k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
   const float2 v[3] = {a0[idx], a1[idx], a2[idx]};
   float re = 0.f, im = 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float f2 = inv * k[a];
+    const float f2 = green_f2(f1, k2, k[a]);
     re += -f2 * v[a].y;   // i f2 (x + i y) = (-f2 y, f2 x)
     im += f2 * v[a].x;
   }

@@ -65,7 +65,8 @@ const int kShapes[][3] = {{8, 8, 64},  {8, 16, 32}, {8, 8, 32}, {4, 8, 64},  {16
 bool shape_ok(int n, const int* t) {
   for (int a = 0; a < 3; ++a)
     if (t[a] > n || n % t[a]) return false;
-  const size_t readout_lds = (size_t)3 * (t[0] + 1) * (t[1] + 1) * (t[2] + 1) * sizeof(float);
+  const size_t rows = (size_t)(t[0] + 1) * (t[1] + 1);
+  const size_t readout_lds = 3 * rows * (t[2] + 1) * sizeof(float) + rows * sizeof(int);
   const size_t paint_lds = (size_t)t[0] * t[1] * t[2] * sizeof(unsigned long long);
   return readout_lds <= 160 * 1024 && paint_lds <= 160 * 1024;
 }

@@ -273,7 +273,7 @@ __device__ __forceinline__ float2 green_mul(float2 v, int ix, int iy, int iz, co
   if (kx == 0 && ky == 0 && iz == 0) return make_float2(0.f, 0.f);
   const float k0 = (float)kx, k1 = (float)ky, k2 = (float)iz;
   const float ka = g.axis == 0 ? k0 : (g.axis == 1 ? k1 : k2);
-  const float f2 = g.f1 / (k0 * k0 + k1 * k1 + k2 * k2) * ka;
+  const float f2 = green_f2(g.f1, k0 * k0 + k1 * k1 + k2 * k2, ka);
   return make_float2(-f2 * v.y, f2 * v.x);
 }
 

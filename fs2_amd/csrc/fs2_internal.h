@@ -113,6 +113,15 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
+// Green x gradient multiplier f2 = f1 k_a / |k|^2 (py/pm.cpp:460-504) with a
+// hardware reciprocal (<= 1 ulp) in place of an IEEE division: this sits in
+// the inner loop of the fused force pass, three times per mode.  The
+// forward and adjoint kernels share it, so M^T = -M holds for the rounded
+// operator too.
+__device__ __forceinline__ float green_f2(float f1, float k2, float ka) {
+  return f1 * __builtin_amdgcn_rcpf(k2) * ka;
+}
+
 // ---- FFT (fft.hip) ------------------------------------------------------------
 // Complex values are in natural [kx][ky][kz] order (kz = 0..n/2).  nb > 1
 // transforms nb meshes stored back to back (grid_floats(n) apart) in one

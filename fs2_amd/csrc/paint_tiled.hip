@@ -40,7 +40,6 @@ constexpr int kBlock = 256;
 constexpr int kTileBlock = 512;
 constexpr int kReadoutBlock = 1024;
 constexpr unsigned kNoKey = 0xffffffffu;
-constexpr double kFix = 4294967296.0;   // 2^32 fixed-point scale
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -320,7 +319,7 @@ __device__ __forceinline__ void tile_source_ranges(const unsigned* __restrict__ 
 template <typename ACC>
 __global__ void __launch_bounds__(kTileBlock)
 k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
-             TileGeom g, float fac, float* __restrict__ grid) {
+             TileGeom g, float fac, float fix, double fix_inv, float* __restrict__ grid) {
   extern __shared__ unsigned char smem[];
   ACC* acc = (ACC*)smem;
   const int tcells = g.tx * g.ty * g.tz;
@@ -374,7 +373,8 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
           if constexpr (sizeof(ACC) == 4) {
             atomicAdd(&acc[li], w);
           } else {
-            atomicAdd(&acc[li], (ACC)(long long)llrint((double)w * kFix));
+            // w <= fac, fix * fac <= 2^30: one rounding to int32 per term
+            atomicAdd(&acc[li], (ACC)(long long)__float2int_rn(w * fix));
           }
         }
       }
@@ -389,10 +389,53 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
     if constexpr (sizeof(ACC) == 4) {
       v = acc[c];
     } else {
-      v = (float)((double)(long long)acc[c] * (1.0 / kFix));
+      v = (float)((double)(long long)acc[c] * fix_inv);
     }
     grid[((int64_t)(x0 + lx) * g.n + (y0 + ly)) * g.ncz + (z0 + lz)] = v - 1.0f;
   }
+}
+
+// Copies the (tx+1) x (ty+1) x (tz+1) halo block of NM meshes starting at
+// cell (x0, y0, z0) (periodic) into LDS, hs[a * hcells + r * hz + lz] with
+// row r = lx * hy + ly.  Row origins are tabulated once in LDS (rowoff, one
+// int per row), so the main copy of the tz interior columns is division
+// free (tz is a power of two) and coalesced along z; the one halo column
+// lz = tz is copied by a second, per-row pass.  Ends with a barrier.
+template <int NM>
+__device__ __forceinline__ void stage_halo(float* hs, int* rowoff, const float* const* ms,
+                                           const TileGeom& g, int x0, int y0, int z0) {
+  const int hy = g.ty + 1, hz = g.tz + 1;
+  const int rows = (g.tx + 1) * hy;
+  const int hcells = rows * hz;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+    const int lx = r / hy, ly = r - lx * hy;
+    int gx = x0 + lx, gy = y0 + ly;
+    if (gx >= g.n) gx -= g.n;
+    if (gy >= g.n) gy -= g.n;
+    rowoff[r] = (gx * g.n + gy) * g.ncz;   // < 2^31 for meshes up to 1024^3
+  }
+  __syncthreads();
+  const int interior = rows << g.lz;
+  for (int i = threadIdx.x; i < interior; i += blockDim.x) {
+    const int r = i >> g.lz, lz = i & (g.tz - 1);
+    const int src = rowoff[r] + z0 + lz;
+#pragma unroll
+    for (int a = 0; a < NM; ++a) hs[a * hcells + r * hz + lz] = ms[a][src];
+  }
+  int gz = z0 + g.tz;
+  if (gz >= g.n) gz -= g.n;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+    const int src = rowoff[r] + gz;
+#pragma unroll
+    for (int a = 0; a < NM; ++a) hs[a * hcells + r * hz + g.tz] = ms[a][src];
+  }
+  __syncthreads();
+}
+
+// LDS bytes of a halo tile of nm meshes plus its row table.
+inline size_t halo_lds(const TileGeom& g, int nm) {
+  const size_t rows = (size_t)(g.tx + 1) * (g.ty + 1);
+  return (size_t)nm * rows * (g.tz + 1) * sizeof(float) + rows * sizeof(int);
 }
 
 // Tiled CIC readout (force_at_particle_locations, py/pm.cpp:142-220, all
@@ -419,19 +462,7 @@ k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offs
   const int Tx = tile / (g.ntz * g.nty);
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
   const float* ms[3] = {m0 + moff, m1 + moff, m2 + moff};
-  for (int c = threadIdx.x; c < hcells; c += blockDim.x) {
-    const int lz = c % hz;
-    const int r = c / hz;
-    const int ly = r % hy, lx = r / hy;
-    int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    if (gx >= g.n) gx -= g.n;
-    if (gy >= g.n) gy -= g.n;
-    if (gz >= g.n) gz -= g.n;
-    const int64_t gi = ((int64_t)gx * g.n + gy) * g.ncz + gz;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) hs[a * hcells + c] = ms[a][gi];
-  }
-  __syncthreads();
+  stage_halo<3>(hs, (int*)(hs + 3 * hcells), ms, g, x0, y0, z0);
   const unsigned b = offsets[(unsigned)tile * 8u], e = offsets[(unsigned)tile * 8u + 8u];
   for (unsigned j = b + threadIdx.x; j < e; j += blockDim.x) {
     const float4 r4 = rec[j];
@@ -610,19 +641,7 @@ k_tile_grad_readout(const float4* __restrict__ rec, const unsigned* __restrict__
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
   const float* ms[4] = {rhobar + moff, FORCES ? m0 + moff : nullptr, FORCES ? m1 + moff : nullptr,
                         FORCES ? m2 + moff : nullptr};
-  for (int c = threadIdx.x; c < hcells; c += blockDim.x) {
-    const int lz = c % hz;
-    const int r = c / hz;
-    const int ly = r % hy, lx = r / hy;
-    int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    if (gx >= g.n) gx -= g.n;
-    if (gy >= g.n) gy -= g.n;
-    if (gz >= g.n) gz -= g.n;
-    const int64_t gi = ((int64_t)gx * g.n + gy) * g.ncz + gz;
-#pragma unroll
-    for (int a = 0; a < NM; ++a) hs[a * hcells + c] = ms[a][gi];
-  }
-  __syncthreads();
+  stage_halo<NM>(hs, (int*)(hs + NM * hcells), ms, g, x0, y0, z0);
   const float dxi = g.dx_inv;
   const unsigned b = offsets[(unsigned)tile * 8u], e = offsets[(unsigned)tile * 8u + 8u];
   for (unsigned j = b + threadIdx.x; j < e; j += blockDim.x) {
@@ -751,14 +770,20 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
            0, (const float*)p.x, p.stride, p.np, (const unsigned*)s.keys, s.cursor, s.rec);
   }
   const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0 * g.nbox;
+  // fixed-point unit 2^-k with fac * 2^k <= 2^30, so every term fits an int32
+  int e = 0;
+  std::frexp((double)std::max(fac, 1e-30f), &e);   // fac < 2^e
+  const int k = std::min(60, 30 - e);
+  const float fix = (float)std::ldexp(1.0, k);
+  const double fix_inv = std::ldexp(1.0, -k);
   if (deterministic) {
     launch("paint_tile_det", bytes, k_tile_paint<unsigned long long>, dim3((unsigned)nblk),
            dim3(kTileBlock), tcells * sizeof(unsigned long long), (const float4*)s.rec,
-           (const unsigned*)s.offsets, g, fac, grid);
+           (const unsigned*)s.offsets, g, fac, fix, fix_inv, grid);
   } else {
     launch("paint_tile", bytes, k_tile_paint<float>, dim3((unsigned)nblk), dim3(kTileBlock),
            tcells * sizeof(float), (const float4*)s.rec, (const unsigned*)s.offsets,
-           g, fac, grid);
+           g, fac, fix, fix_inv, grid);
   }
 }
 
@@ -767,7 +792,7 @@ void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, 
   const TileGeom g = make_geom(n, boxsize, o, np);
   check_scratch(g, np, s);
   const int64_t nblk = (int64_t)g.nbox * g.ntiles;
-  const size_t lds = (size_t)3 * (g.tx + 1) * (g.ty + 1) * (g.tz + 1) * sizeof(float);
+  const size_t lds = halo_lds(g, 3);
   check_lds(lds, "readout");
   const double bytes = (double)np * 24.0 + 3.0 * 4.0 * (double)n * n * n * g.nbox;
   launch("readout_tile", bytes, k_tile_readout, dim3((unsigned)nblk), dim3(kReadoutBlock), lds,
@@ -799,16 +824,15 @@ void grad_readout_tiled(const float* rhobar, float fac, const float* const F[3],
   const TileGeom g = make_geom(n, boxsize, o, np);
   check_scratch(g, np, s);
   const int64_t nblk = (int64_t)g.nbox * g.ntiles;
-  const size_t hcells = (size_t)(g.tx + 1) * (g.ty + 1) * (g.tz + 1);
-  check_lds((F ? 4 : 1) * hcells * sizeof(float), "gradient readout");
+  check_lds(halo_lds(g, F ? 4 : 1), "gradient readout");
   if (F) {
     launch("adj_grad_tile4", (double)np * 40.0 + 16.0 * (double)n * n * n * g.nbox,
-           k_tile_grad_readout<true>, dim3((unsigned)nblk), dim3(512), 4 * hcells * sizeof(float),
+           k_tile_grad_readout<true>, dim3((unsigned)nblk), dim3(512), halo_lds(g, 4),
            (const float4*)s.rec, (const unsigned*)s.offsets, g, rhobar, fac, F[0], F[1], F[2],
            fbar, xbar, stride);
   } else {
     launch("adj_grad_tile1", (double)np * 28.0 + 4.0 * (double)n * n * n * g.nbox,
-           k_tile_grad_readout<false>, dim3((unsigned)nblk), dim3(512), hcells * sizeof(float),
+           k_tile_grad_readout<false>, dim3((unsigned)nblk), dim3(512), halo_lds(g, 1),
            (const float4*)s.rec, (const unsigned*)s.offsets, g, rhobar, fac, nullptr, nullptr,
            nullptr, nullptr, xbar, stride);
   }

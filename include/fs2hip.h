@@ -53,9 +53,10 @@ enum {
 enum {
   FS2_PAINT_ATOMIC = 0,   /* tiled, LDS FP32 atomics; summation order varies
                              run to run                                     */
-  FS2_PAINT_SORTED = 1,   /* default: tiled, per-tile segmented sums in int64
-                             fixed point (2^-32); bitwise reproducible and,
-                             on gfx950, faster than FP32 LDS atomics        */
+  FS2_PAINT_SORTED = 1,   /* default: tiled, per-tile sums in int64 fixed
+                             point (unit 2^-k, fac 2^k <= 2^30: 2^-27 at
+                             pm_factor 2); bitwise reproducible and, on
+                             gfx950, faster than FP32 LDS atomics           */
   FS2_PAINT_GLOBAL = 2    /* one thread per particle, 8 global FP32 atomics
                              onto a mesh cleared to -1 (pm.cpp's loop as is) */
 };
