@@ -356,10 +356,14 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int 
     dk[i] = valid ? src[base + j * es] : make_float2(0.f, 0.f);
   }
   fft_regs<N, L>(dk, s + p, t, stw);
-  float2* dsts[3] = {dst0, dst1, dst2};
+  // Along an x pencil ky and kz are constant, so the y and z force spectra
+  // are i ky and i kz times one transform Phi = IFFT_x(f1/|k|^2 delta_k):
+  // two inverse transforms instead of three.  Pass 0 makes the x force,
+  // pass 1 Phi and from it the y and z forces.
+  const int kyi = o <= N / 2 ? o : o - N;
+  const float ky = (float)kyi, kz = (float)c;
 #pragma unroll 1
-  for (int a = 0; a < 3; ++a) {
-    const GreenArg g{a, N, f1};
+  for (int pass = 0; pass < 2; ++pass) {
     // Opaque copies of the thread coordinates: without them the compiler
     // hoists every per-stage LDS address out of this loop and keeps ~100
     // of them live, halving occupancy.
@@ -368,17 +372,30 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int 
     float2 x[N / T];
 #pragma unroll
     for (int i = 0; i < N / T; ++i) {
-      float2 v = green_mul(dk[i], tl + i * T, o, c, g);
-      v.y = -v.y;
-      x[i] = v;
+      const int j = tl + i * T;
+      const int kxi = j <= N / 2 ? j : j - N;
+      const float kx = (float)kxi;
+      const float k2 = kx * kx + ky * ky + kz * kz;
+      // green_f2 with k_a = kx (pass 0) or 1 (pass 1); k = 0 -> 0
+      const float f2 = k2 > 0.f ? green_f2(f1, k2, pass == 0 ? kx : 1.f) : 0.f;
+      // conj(i f2 dk): the inverse transform runs as a conjugated forward one
+      x[i] = make_float2(-f2 * dk[i].y, -(f2 * dk[i].x));
     }
     fft_regs<N, L>(x, s + pl, tl, stw);
-    float2* dst = dsts[a];
+    if (pass == 0) {
 #pragma unroll
-    for (int i = 0; i < N / T; ++i) {
-      float2 v = x[i];
-      v.y = -v.y;
-      if (valid) dst[base + (tl + i * T) * es] = v;
+      for (int i = 0; i < N / T; ++i)
+        if (valid) dst0[base + (tl + i * T) * es] = make_float2(x[i].x, -x[i].y);
+    } else {
+      // x holds conj(IFFT(i f1/|k|^2 delta_k)) = conj(i Phi); F_a = k_a (i Phi)
+#pragma unroll
+      for (int i = 0; i < N / T; ++i) {
+        const float2 iphi = make_float2(x[i].x, -x[i].y);
+        if (valid) {
+          dst1[base + (tl + i * T) * es] = make_float2(ky * iphi.x, ky * iphi.y);
+          dst2[base + (tl + i * T) * es] = make_float2(kz * iphi.x, kz * iphi.y);
+        }
+      }
     }
   }
 }
