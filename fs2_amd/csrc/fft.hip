@@ -328,7 +328,7 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
 // corresponding force mesh.  Saves the delta_k write and its three reads.
 template <int N, int P>
 __global__ void __launch_bounds__(Plan<N>::T * P)
-k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int ncols,
+k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, int ncols,
                int64_t elem_stride, int64_t outer_stride, int nchunks, int nblocks,
                const float2* __restrict__ tw, float f1, int64_t bstride) {
   constexpr int T = Plan<N>::T;
@@ -340,7 +340,6 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int 
   src += boff;
   dst0 += boff;
   dst1 += boff;
-  dst2 += boff;
   const int tid = threadIdx.x;
   const int p = tid % P, t = tid / P;
   const int bid = xcd_remap(blockIdx.x, nblocks);
@@ -357,9 +356,9 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int 
   }
   fft_regs<N, L>(dk, s + p, t, stw);
   // Along an x pencil ky and kz are constant, so the y and z force spectra
-  // are i ky and i kz times one transform Phi = IFFT_x(f1/|k|^2 delta_k):
-  // two inverse transforms instead of three.  Pass 0 makes the x force,
-  // pass 1 Phi and from it the y and z forces.
+  // are ky and kz times one transform G = IFFT_x(i f1/|k|^2 delta_k): two
+  // inverse transforms instead of three, and two meshes written instead of
+  // three.  Pass 0 writes the x force to dst0, pass 1 writes G to dst1.
   const int kyi = o <= N / 2 ? o : o - N;
   const float ky = (float)kyi, kz = (float)c;
 #pragma unroll 1
@@ -387,15 +386,63 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, float2* dst2, int 
       for (int i = 0; i < N / T; ++i)
         if (valid) dst0[base + (tl + i * T) * es] = make_float2(x[i].x, -x[i].y);
     } else {
-      // x holds conj(IFFT(i f1/|k|^2 delta_k)) = conj(i Phi); F_a = k_a (i Phi)
+      // x holds conj(IFFT(i f1/|k|^2 delta_k)) = conj(i Phi); the y pass
+      // k_fft_y_inv_dual turns i Phi into the y and z forces
 #pragma unroll
-      for (int i = 0; i < N / T; ++i) {
-        const float2 iphi = make_float2(x[i].x, -x[i].y);
-        if (valid) {
-          dst1[base + (tl + i * T) * es] = make_float2(ky * iphi.x, ky * iphi.y);
-          dst2[base + (tl + i * T) * es] = make_float2(kz * iphi.x, kz * iphi.y);
-        }
-      }
+      for (int i = 0; i < N / T; ++i)
+        if (valid) dst1[base + (tl + i * T) * es] = make_float2(x[i].x, -x[i].y);
+    }
+  }
+}
+
+// Inverse y pass that splits G = IFFT_x(i f1/|k|^2 delta_k) (the second
+// output of k_fft_x_green3) into the y and z forces: along a y pencil kz is
+// constant, so F_y = IFFT_y(ky G) and F_z = kz IFFT_y(G).  G is read once;
+// F_y replaces it in place and F_z goes to dst2.
+template <int N, int P>
+__global__ void __launch_bounds__(Plan<N>::T * P)
+k_fft_y_inv_dual(float2* g1, float2* dst2, int ncols, int64_t elem_stride, int64_t outer_stride,
+                 int nchunks, int nblocks, const float2* __restrict__ tw, int64_t bstride) {
+  constexpr int T = Plan<N>::T;
+  using L = RowsLayout<P>;
+  __shared__ float2 s[L::rows(N) * P];
+  __shared__ float2 stw[N];
+  stage_twiddles<N>(stw, tw);
+  g1 += blockIdx.y * bstride;
+  dst2 += blockIdx.y * bstride;
+  const int tid = threadIdx.x;
+  const int p = tid % P, t = tid / P;
+  const int bid = xcd_remap(blockIdx.x, nblocks);
+  const int o = bid / nchunks;
+  const int c = (bid - o * nchunks) * P + p;
+  const bool valid = c < ncols;
+  const int base = o * (int)outer_stride + c;
+  const int es = (int)elem_stride;
+  const float kz = (float)c;
+  float2 gk[N / T];
+#pragma unroll
+  for (int i = 0; i < N / T; ++i)
+    gk[i] = valid ? g1[base + (t + i * T) * es] : make_float2(0.f, 0.f);
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    int tl = t, pl = p;
+    asm volatile("" : "+v"(tl), "+v"(pl));
+    float2 x[N / T];
+#pragma unroll
+    for (int i = 0; i < N / T; ++i) {
+      const int j = tl + i * T;
+      const float m = pass == 0 ? (float)(j <= N / 2 ? j : j - N) : 1.f;   // ky or 1
+      x[i] = make_float2(m * gk[i].x, -(m * gk[i].y));
+    }
+    fft_regs<N, L>(x, s + pl, tl, stw);
+    if (pass == 0) {
+#pragma unroll
+      for (int i = 0; i < N / T; ++i)
+        if (valid) g1[base + (tl + i * T) * es] = make_float2(x[i].x, -x[i].y);
+    } else {
+#pragma unroll
+      for (int i = 0; i < N / T; ++i)
+        if (valid) dst2[base + (tl + i * T) * es] = make_float2(kz * x[i].x, -(kz * x[i].y));
     }
   }
 }
@@ -587,16 +634,21 @@ struct Impl {
       const int ncols = n / 2 + 1;
       const int nchunks = (ncols + P - 1) / P;
       const int nblocks = n * nchunks;
-      launch("fft_x_fwd_green3", 4.0 * grid_bytes() * nb, k_fft_x_green3<N, P>,
+      launch("fft_x_fwd_green3", 3.0 * grid_bytes() * nb, k_fft_x_green3<N, P>,
              dim3(nblocks, nb), dim3(Plan<N>::T * P), 0, (const float2*)c, (float2*)out[0],
-             (float2*)out[1], (float2*)out[2], ncols, (int64_t)n * nh, nh, nchunks, nblocks,
-             twiddles(N), f1, bstride());
+             (float2*)out[1], ncols, (int64_t)n * nh, nh, nchunks, nblocks, twiddles(N), f1,
+             bstride());
     }
-    for (int a = 0; a < 3; ++a) {
-      float2* o = (float2*)out[a];
-      pencil<true, false>("fft_y_inv", o, o, nh, (int64_t)n * nh, g, nb);
-      z_c2r(out[a], nb);
+    pencil<true, false>("fft_y_inv", (float2*)out[0], (float2*)out[0], nh, (int64_t)n * nh, g, nb);
+    {
+      const int ncols = n / 2 + 1;
+      const int nchunks = (ncols + P - 1) / P;
+      const int nblocks = n * nchunks;
+      launch("fft_y_inv_dual", 3.0 * grid_bytes() * nb, k_fft_y_inv_dual<N, P>,
+             dim3(nblocks, nb), dim3(Plan<N>::T * P), 0, (float2*)out[1], (float2*)out[2],
+             ncols, nh, (int64_t)n * nh, nchunks, nblocks, twiddles(N), bstride());
     }
+    for (int a = 0; a < 3; ++a) z_c2r(out[a], nb);
   }
 
   static void green_c2r(const float* dk, float* dst, int axis, float f1) {
