@@ -112,38 +112,6 @@ __global__ void k_grad_readout(const float* __restrict__ x, int64_t stride, int6
   xbar[i + 2 * stride] += gz;
 }
 
-// rhobar_k = -sum_a (i f2_a) A_a(k) on the half spectrum (PM convention:
-// indices <= n/2 positive, kz = 0..n/2, f2_a = f1 k_a / |k|^2), in place in
-// a0's spectrum.
-__global__ void k_green_sum_adj(float2* a0, const float2* __restrict__ a1,
-                                const float2* __restrict__ a2, int n, int cpitch, float f1) {
-  const int nh = n / 2 + 1;
-  const int64_t total = (int64_t)n * n * nh;
-  const int64_t mi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (mi >= total) return;
-  const int iz = (int)(mi % nh);
-  const int64_t rxy = mi / nh;
-  const int iy = (int)(rxy % n), ix = (int)(rxy / n);
-  const int64_t idx = rxy * cpitch + iz;
-  const int kx = ix <= n / 2 ? ix : ix - n;
-  const int ky = iy <= n / 2 ? iy : iy - n;
-  if (kx == 0 && ky == 0 && iz == 0) {
-    a0[idx] = make_float2(0.f, 0.f);
-    return;
-  }
-  const float k[3] = {(float)kx, (float)ky, (float)iz};
-  const float k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-  const float2 v[3] = {a0[idx], a1[idx], a2[idx]};
-  float re = 0.f, im = 0.f;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float f2 = green_f2(f1, k2, k[a]);
-    re += -f2 * v[a].y;   // i f2 (x + i y) = (-f2 y, f2 x)
-    im += f2 * v[a].x;
-  }
-  a0[idx] = make_float2(-re, -im);
-}
-
 // One COLA step backwards (py/cola.cpp:79-88, 117-124 transposed):
 //   vtot = vbar + dt xbar; dx1bar += da1 xbar + c q1 vtot;
 //   dx2bar += da2 xbar + c q2 vtot; fbar = c vtot; vbar = vtot
@@ -298,13 +266,6 @@ void adj_grad_readout(const float* x, int64_t stride, int64_t np, const float* r
          dim3(blocks_for(np)), dim3(kBlock), 0, x, stride, np, rhobar, fac,
          F ? F[0] : nullptr, F ? F[1] : nullptr, F ? F[2] : nullptr, fbar, xbar, n,
          grid_rpitch(n), (float)n / boxsize);
-}
-
-void adj_green_sum(float* const a[3], int n, float f1) {
-  const int64_t total = (int64_t)n * n * (n / 2 + 1);
-  launch("adj_green_sum", (double)total * 32.0, k_green_sum_adj, dim3(blocks_for(total)),
-         dim3(kBlock), 0, (float2*)a[0], (const float2*)a[1], (const float2*)a[2], n,
-         grid_cpitch(n), f1);
 }
 
 void adj_cola_step(const float* xbar, float* vbar, float* dx1bar, float* dx2bar, float* fbar,

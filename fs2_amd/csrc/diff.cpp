@@ -344,9 +344,7 @@ int fs2_diff_backward(fs2_diff* d, const float* delta_bar, float* noise_bar) {
         for (float* g : d->P) fill_grid(g, nm, 0.0f);
         adj_paint_values3(d->x_at(i), d->fbar, np, np, d->P, nm, d->boxsize_f);
       }
-      for (float* g : d->P) fft_r2c_3d(g, nm);
-      adj_green_sum(d->P, nm, d->f1);
-      fft_c2r_3d(d->P[0], nm);
+      fft_adjoint_green(d->P, nm, d->f1);
       grad_readout(d, i, d->P[0], F, d->fbar);
     }
 

@@ -447,6 +447,101 @@ k_fft_y_inv_dual(float2* g1, float2* dst2, int ncols, int64_t elem_stride, int64
   }
 }
 
+// ---- adjoint of the force-mesh operator ------------------------------------
+// rho_bar = -sum_a M_a P_a with M_a = c2r . (i f1 k_a/|k|^2) . r2c needs the
+// spectrum S = sum_a k_a P_a^.  k_z is constant along y pencils and k_y, k_z
+// along x pencils, so S is formed while transforming: a dual forward y pass
+// makes Q = k_y Y(P_y) + k_z Y(P_z), a fused x pass forms S = k_x X(P_x) +
+// X(Q), multiplies by -i f1/|k|^2 and transforms back along x.
+
+// Q = ky Y(a) + kz Y(b) along y pencils (a, b z-transformed); Q overwrites a.
+template <int N, int P>
+__global__ void __launch_bounds__(Plan<N>::T * P)
+k_fft_y_fwd_dual(float2* a, const float2* b, int ncols, int64_t elem_stride,
+                 int64_t outer_stride, int nchunks, int nblocks, const float2* __restrict__ tw) {
+  constexpr int T = Plan<N>::T;
+  using L = RowsLayout<P>;
+  __shared__ float2 s[L::rows(N) * P];
+  __shared__ float2 stw[N];
+  stage_twiddles<N>(stw, tw);
+  const int tid = threadIdx.x;
+  const int p = tid % P, t = tid / P;
+  const int bid = xcd_remap(blockIdx.x, nblocks);
+  const int o = bid / nchunks;
+  const int c = (bid - o * nchunks) * P + p;
+  const bool valid = c < ncols;
+  const int base = o * (int)outer_stride + c;
+  const int es = (int)elem_stride;
+  const float kz = (float)c;
+  float2 ya[N / T];
+#pragma unroll
+  for (int i = 0; i < N / T; ++i) ya[i] = valid ? a[base + (t + i * T) * es] : make_float2(0.f, 0.f);
+  fft_regs<N, L>(ya, s + p, t, stw);
+  int tl = t, pl = p;
+  asm volatile("" : "+v"(tl), "+v"(pl));
+  float2 yb[N / T];
+#pragma unroll
+  for (int i = 0; i < N / T; ++i)
+    yb[i] = valid ? b[base + (tl + i * T) * es] : make_float2(0.f, 0.f);
+  fft_regs<N, L>(yb, s + pl, tl, stw);
+#pragma unroll
+  for (int i = 0; i < N / T; ++i) {
+    const int j = tl + i * T;
+    const float ky = (float)(j <= N / 2 ? j : j - N);
+    if (valid)
+      a[base + j * es] = make_float2(ky * ya[i].x + kz * yb[i].x, ky * ya[i].y + kz * yb[i].y);
+  }
+}
+
+// R = IFFT_x(-i f1/|k|^2 (kx X(a) + X(q))) along x pencils; R overwrites a.
+template <int N, int P>
+__global__ void __launch_bounds__(Plan<N>::T * P)
+k_fft_x_div_green(float2* a, const float2* q, int ncols, int64_t elem_stride,
+                  int64_t outer_stride, int nchunks, int nblocks, const float2* __restrict__ tw,
+                  float f1) {
+  constexpr int T = Plan<N>::T;
+  using L = RowsLayout<P>;
+  __shared__ float2 s[L::rows(N) * P];
+  __shared__ float2 stw[N];
+  stage_twiddles<N>(stw, tw);
+  const int tid = threadIdx.x;
+  const int p = tid % P, t = tid / P;
+  const int bid = xcd_remap(blockIdx.x, nblocks);
+  const int o = bid / nchunks;
+  const int c = (bid - o * nchunks) * P + p;
+  const bool valid = c < ncols;
+  const int base = o * (int)outer_stride + c;
+  const int es = (int)elem_stride;
+  const float ky = (float)(o <= N / 2 ? o : o - N), kz = (float)c;
+  float2 xa[N / T];
+#pragma unroll
+  for (int i = 0; i < N / T; ++i) xa[i] = valid ? a[base + (t + i * T) * es] : make_float2(0.f, 0.f);
+  fft_regs<N, L>(xa, s + p, t, stw);
+  int tl = t, pl = p;
+  asm volatile("" : "+v"(tl), "+v"(pl));
+  float2 x[N / T];
+#pragma unroll
+  for (int i = 0; i < N / T; ++i)
+    x[i] = valid ? q[base + (tl + i * T) * es] : make_float2(0.f, 0.f);
+  fft_regs<N, L>(x, s + pl, tl, stw);
+  int tm = tl, pm = pl;
+  asm volatile("" : "+v"(tm), "+v"(pm));
+#pragma unroll
+  for (int i = 0; i < N / T; ++i) {
+    const int j = tm + i * T;
+    const float kx = (float)(j <= N / 2 ? j : j - N);
+    const float k2 = kx * kx + ky * ky + kz * kz;
+    const float g = k2 > 0.f ? green_f2(f1, k2, 1.f) : 0.f;
+    const float2 S = make_float2(kx * xa[i].x + x[i].x, kx * xa[i].y + x[i].y);
+    // conj(-i g S) = (g S.y, g S.x): the inverse runs as a conjugated forward FFT
+    x[i] = make_float2(g * S.y, g * S.x);
+  }
+  fft_regs<N, L>(x, s + pm, tm, stw);
+#pragma unroll
+  for (int i = 0; i < N / T; ++i)
+    if (valid) a[base + (tm + i * T) * es] = make_float2(x[i].x, -x[i].y);
+}
+
 // z pass, r2c: each row of n = 2M reals (padded to M+1 complex) is read as M
 // complex z[m] = x[2m] + i x[2m+1], transformed, and split:
 //   X[k] = E - i W^k O,  E = (Z[k] + conj Z[M-k])/2,  O = (Z[k] - conj Z[M-k])/2,
@@ -651,6 +746,26 @@ struct Impl {
     for (int a = 0; a < 3; ++a) z_c2r(out[a], nb);
   }
 
+  // p[0..2]: real meshes P_a on entry; p[0] holds rho_bar = -sum_a M_a P_a
+  // on return, p[1] and p[2] are clobbered.
+  static void adjoint_green(float* const pm[3], float f1) {
+    const int64_t nh = grid_cpitch(n);
+    const GreenArg g{0, n, 0.f};
+    for (int a = 0; a < 3; ++a) z_r2c(pm[a], 1);
+    pencil<false, false>("fft_y_fwd", (float2*)pm[0], (float2*)pm[0], nh, (int64_t)n * nh, g, 1);
+    const int ncols = n / 2 + 1;
+    const int nchunks = (ncols + P - 1) / P;
+    const int nblocks = n * nchunks;
+    launch("fft_y_fwd_dual", 3.0 * grid_bytes(), k_fft_y_fwd_dual<N, P>, dim3(nblocks),
+           dim3(Plan<N>::T * P), 0, (float2*)pm[1], (const float2*)pm[2], ncols, nh,
+           (int64_t)n * nh, nchunks, nblocks, twiddles(N));
+    launch("fft_x_div_green", 3.0 * grid_bytes(), k_fft_x_div_green<N, P>, dim3(nblocks),
+           dim3(Plan<N>::T * P), 0, (float2*)pm[0], (const float2*)pm[1], ncols,
+           (int64_t)n * nh, nh, nchunks, nblocks, twiddles(N), f1);
+    pencil<true, false>("fft_y_inv", (float2*)pm[0], (float2*)pm[0], nh, (int64_t)n * nh, g, 1);
+    z_c2r(pm[0], 1);
+  }
+
   static void green_c2r(const float* dk, float* dst, int axis, float f1) {
     const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)dst;
@@ -704,6 +819,10 @@ void fft_c2r_3d(float* grid, int n, int nb) {
 
 void fft_force_meshes(float* mesh, float* const out[3], int n, float f1, int nb) {
   dispatch(n, [&](auto impl) { decltype(impl)::force_meshes(mesh, out, f1, nb); });
+}
+
+void fft_adjoint_green(float* const p[3], int n, float f1) {
+  dispatch(n, [&](auto impl) { decltype(impl)::adjoint_green(p, f1); });
 }
 
 void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f1) {

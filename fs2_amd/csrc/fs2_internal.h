@@ -138,6 +138,9 @@ void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f
 // gradient spectra, then y and z c2r passes of each force mesh.  The density
 // mesh is left holding its y-transformed intermediate.
 void fft_force_meshes(float* mesh, float* const out[3], int n, float f1, int nb = 1);
+// Transpose of the force-mesh operator: p[0] <- -sum_a M_a p[a] (real meshes,
+// M_a = c2r . (i f1 k_a/|k|^2) . r2c); p[1] and p[2] are clobbered.
+void fft_adjoint_green(float* const p[3], int n, float f1);
 
 // ---- PM kernels (pm_kernels.hip) -----------------------------------------------
 struct PosArgs {
@@ -246,7 +249,6 @@ void adj_paint_values3(const float* x, const float* val, int64_t stride, int64_t
 void adj_grad_readout(const float* x, int64_t stride, int64_t np, const float* rhobar, float fac,
                       const float* const F[3], const float* fbar, float* xbar, int n,
                       float boxsize);
-void adj_green_sum(float* const a[3], int n, float f1);
 void adj_cola_step(const float* xbar, float* vbar, float* dx1bar, float* dx2bar, float* fbar,
                    int64_t n3, float dt, float da1, float da2, float c, float q1, float q2);
 void adj_axpy(float* y, const float* x, float a, int64_t n);

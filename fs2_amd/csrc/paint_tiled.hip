@@ -516,25 +516,24 @@ k_absmax3(const float* __restrict__ v, int64_t stride, int64_t np, unsigned* __r
   }
 }
 
-// Fixed-point scale of one value plane: every cell sum is bounded by
-// sum_p |v_p| <= np * amax < 2^e, so scale = 2^(62 - e) keeps int64 sums
-// exact; a term's rounding error is 2^(e - 62), i.e. np * amax * 2^-62.
-__device__ __forceinline__ float value_scale(unsigned amax_bits, float npf) {
+// Fixed-point scale of one value plane: a term w v (w <= 1) is bounded by
+// amax < 2^e, so scale = 2^(30 - e) makes every term an int32 (one rounding,
+// 2^(e - 31) at most) and the int64 cell sums exact.
+__device__ __forceinline__ float value_scale(unsigned amax_bits) {
   const float amax = __uint_as_float(amax_bits);
   if (!(amax > 0.f)) return 1.f;
-  const float bound = amax * npf;
-  if (!(bound < 3.0e38f)) return 0x1p-66f;   // inf / nan values: results are not finite anyway
+  if (!(amax < 3.0e38f)) return 0x1p-98f;   // inf / nan values: results are not finite anyway
   int e;
-  frexpf(bound, &e);
-  return ldexpf(1.f, 62 - e);
+  frexpf(amax, &e);
+  return ldexpf(1.f, 30 - e);
 }
 
 // Transpose of the three-field readout: out_a(c) = sum_p W(x_p - c) v_a(p),
 // owner-computes per tile like k_tile_paint, with int64 fixed-point LDS sums
 // (bitwise reproducible).  Every cell of the mesh is written.
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
-                     TileGeom g, const float* __restrict__ vals, int64_t vstride, float npf,
+                     TileGeom g, const float* __restrict__ vals, int64_t vstride,
                      const unsigned* __restrict__ amax, float* __restrict__ o0,
                      float* __restrict__ o1, float* __restrict__ o2) {
   extern __shared__ unsigned char smem[];
@@ -554,7 +553,7 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
   float sc[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) sc[a] = value_scale(amax[a], npf);
+  for (int a = 0; a < 3; ++a) sc[a] = value_scale(amax[a]);
 
   __shared__ unsigned offs[8][9];
   __shared__ unsigned rb[16], rpre[17];
@@ -593,8 +592,8 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
           const int li = (((lx[ex] << g.ly) + ly[ey]) << g.lz) + lz[ez];
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            const long long q = (long long)rintf(w * val[a] * sc[a]);
-            atomicAdd(&acc[a * tcells + li], (unsigned long long)q);
+            const int q = __float2int_rn(w * val[a] * sc[a]);
+            atomicAdd(&acc[a * tcells + li], (unsigned long long)(long long)q);
           }
         }
       }
@@ -619,7 +618,7 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
 // and its +1 halo in LDS.  Same FP32 expression order as the gather kernel
 // of adjoint_kernels.hip, so the two agree bitwise.
 template <bool FORCES>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 k_tile_grad_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
                     TileGeom g, const float* __restrict__ rhobar, float fac,
                     const float* __restrict__ m0, const float* __restrict__ m1,
@@ -812,9 +811,9 @@ void paint_values3_tiled(const float* vals, int64_t vstride, int64_t np, float* 
            dim3(std::min<unsigned>(blocks_for(np), 1024u)), dim3(kBlock), 0, vals, vstride, np,
            s.amax);
   launch("adj_paint_tile3", (double)np * 28.0 + 12.0 * (double)n * n * n * g.nbox,
-         k_tile_paint_values3, dim3((unsigned)nblk), dim3(512),
+         k_tile_paint_values3, dim3((unsigned)nblk), dim3(1024),
          3 * tcells * sizeof(unsigned long long), (const float4*)s.rec,
-         (const unsigned*)s.offsets, g, vals, vstride, (float)np, (const unsigned*)s.amax, out[0],
+         (const unsigned*)s.offsets, g, vals, vstride, (const unsigned*)s.amax, out[0],
          out[1], out[2]);
 }
 
@@ -827,12 +826,12 @@ void grad_readout_tiled(const float* rhobar, float fac, const float* const F[3],
   check_lds(halo_lds(g, F ? 4 : 1), "gradient readout");
   if (F) {
     launch("adj_grad_tile4", (double)np * 40.0 + 16.0 * (double)n * n * n * g.nbox,
-           k_tile_grad_readout<true>, dim3((unsigned)nblk), dim3(512), halo_lds(g, 4),
+           k_tile_grad_readout<true>, dim3((unsigned)nblk), dim3(1024), halo_lds(g, 4),
            (const float4*)s.rec, (const unsigned*)s.offsets, g, rhobar, fac, F[0], F[1], F[2],
            fbar, xbar, stride);
   } else {
     launch("adj_grad_tile1", (double)np * 28.0 + 4.0 * (double)n * n * n * g.nbox,
-           k_tile_grad_readout<false>, dim3((unsigned)nblk), dim3(512), halo_lds(g, 1),
+           k_tile_grad_readout<false>, dim3((unsigned)nblk), dim3(1024), halo_lds(g, 1),
            (const float4*)s.rec, (const unsigned*)s.offsets, g, rhobar, fac, nullptr, nullptr,
            nullptr, nullptr, xbar, stride);
   }
