@@ -55,6 +55,7 @@ struct fs2_diff {
   float* F[3] = {};           // force meshes (recompute mode)
   float* P[3] = {};
   float* Fs = nullptr;        // store mode: [nstep][3] force meshes
+  fs2::TileOpts tile;         // paint / readout tile extents
   fs2::PaintScratch paint;    // shared bins
   std::vector<fs2::PaintScratch> bins;   // store mode: per-step rec / offsets
 
@@ -108,7 +109,7 @@ bool tiled(const fs2_diff* d) { return d->variant == FS2_PAINT_SORTED; }
 void density(fs2_diff* d, int i, const PaintScratch& bins, bool wrap) {
   const PosArgs pos{d->x_at(i), d->np, d->np};
   if (tiled(d)) {
-    paint_tiled(pos, d->mesh, d->nm, d->boxsize_f, d->fac, wrap, true, bins);
+    paint_tiled(pos, d->mesh, d->nm, d->boxsize_f, d->fac, wrap, true, bins, d->tile);
     if (&bins == &d->paint) d->shared_bins_step = i;
   } else {
     fill_grid(d->mesh, d->nm, -1.0f);
@@ -120,7 +121,7 @@ void grad_readout(fs2_diff* d, int i, const float* rhobar, const float* const F[
                   const float* fbar) {
   if (tiled(d))
     grad_readout_tiled(rhobar, d->fac, F, fbar, d->xbar, d->np, d->np, d->nm, d->boxsize_f,
-                       d->bins_of(i));
+                       d->bins_of(i), d->tile);
   else
     adj_grad_readout(d->x_at(i), d->np, d->np, rhobar, d->fac, F, fbar, d->xbar, d->nm,
                      d->boxsize_f);
@@ -194,7 +195,7 @@ int fs2_diff_create(int nc_lpt, int nc_pm, double boxsize, double a0, double a1,
       const size_t lbytes = grid_floats(nc_lpt) * sizeof(float);
       const size_t mbytes = grid_floats(nc_pm) * sizeof(float);
       const size_t pbytes = 3 * (size_t)d->np * sizeof(float);
-      const size_t nbins = paint_tiled_bins(nc_pm);
+      const size_t nbins = paint_tiled_bins(nc_pm, d->tile);
       const size_t store_bytes =
           (size_t)nstep * (3 * mbytes + (size_t)d->np * sizeof(float4) + (nbins + 1) * 4);
       if (checkpoint == FS2_DIFF_AUTO) {
@@ -296,7 +297,7 @@ int fs2_diff_forward(fs2_diff* d, const float* noise, float* delta) {
       density(d, i, d->bins_of(i), true);   // the checkpoint is the wrapped position
       fft_force_meshes(d->mesh, F, nm, d->f1);
       if (tiled(d))
-        readout_tiled(F[0], F[1], F[2], d->f, np, np, nm, d->boxsize_f, d->bins_of(i));
+        readout_tiled(F[0], F[1], F[2], d->f, np, np, nm, d->boxsize_f, d->bins_of(i), d->tile);
       else
         readout3(PosArgs{d->x_at(i), np, np}, F[0], F[1], F[2], d->f, nm, d->boxsize_f);
       cola_kick(d->v, d->f, d->dx1, d->dx2, np, np, m15om, s.kf, s.q1, s.q2);
@@ -339,7 +340,7 @@ int fs2_diff_backward(fs2_diff* d, const float* delta_bar, float* noise_bar) {
       }
       // rhobar = M^T (paint of fbar) = -sum_a M_a P_a
       if (tiled(d)) {
-        paint_values3_tiled(d->fbar, np, np, d->P, nm, d->boxsize_f, d->bins_of(i));
+        paint_values3_tiled(d->fbar, np, np, d->P, nm, d->boxsize_f, d->bins_of(i), d->tile);
       } else {
         for (float* g : d->P) fill_grid(g, nm, 0.0f);
         adj_paint_values3(d->x_at(i), d->fbar, np, np, d->P, nm, d->boxsize_f);

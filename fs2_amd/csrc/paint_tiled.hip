@@ -263,53 +263,53 @@ __global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t n
 
 // Source tiles S = T - d, d in {0,1}^3 (periodic).  A particle of S
 // reaches T only by spilling along every axis where S != T, so S
-// contributes the classes c with (c & need) == need.  All 8 x 9 bin
-// offsets are loaded at once; the needed class runs are merged into at
-// most 15 contiguous ranges [rb[r], rb[r] + rpre[r+1] - rpre[r]) of the
-// record array, which all threads of the block then stride over.
-// Block-cooperative; ends with a barrier.
+// contributes the classes c with (c & need) == need.  Wavefront 0 examines
+// the 64 (d, class) pairs in parallel, one per lane, and compacts the
+// non-empty bin ranges: range r covers records [rb[r], rb[r] + rpre[r+1] -
+// rpre[r]), rpre is their exclusive prefix sum and rpre[nranges] the total.
+// rb needs 64 entries, rpre 65.  Block-cooperative; ends with a barrier.
 __device__ __forceinline__ void tile_source_ranges(const unsigned* __restrict__ offsets,
                                                    const TileGeom& g, int Tx, int Ty, int Tz,
-                                                   unsigned (*offs)[9], unsigned* rb,
-                                                   unsigned* rpre, int* nranges) {
-  if (threadIdx.x < 72) {
-    const int d = threadIdx.x / 9, cl = threadIdx.x % 9;
-    const int Sx = (Tx - ((d >> 2) & 1) + g.ntx) % g.ntx;
-    const int Sy = (Ty - ((d >> 1) & 1) + g.nty) % g.nty;
-    const int Sz = (Tz - (d & 1) + g.ntz) % g.ntz;
-    offs[d][cl] = offsets[(unsigned)((Sx * g.nty + Sy) * g.ntz + Sz) * 8u + cl];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int S_of[8];
-    int nr = 0;
-    unsigned total = 0;
-    for (int d = 0; d < 8; ++d) {
-      const int Sx = (Tx - ((d >> 2) & 1) + g.ntx) % g.ntx;
-      const int Sy = (Ty - ((d >> 1) & 1) + g.nty) % g.nty;
-      const int Sz = (Tz - (d & 1) + g.ntz) % g.ntz;
-      S_of[d] = (Sx * g.nty + Sy) * g.ntz + Sz;
-      bool seen = false;
-      for (int e = 0; e < d; ++e) seen |= (S_of[e] == S_of[d]);
-      if (seen) continue;
-      const unsigned need = (Sx != Tx ? 4u : 0u) | (Sy != Ty ? 2u : 0u) | (Sz != Tz ? 1u : 0u);
-      for (unsigned cl = 0; cl < 8; ++cl) {
-        if ((cl & need) != need) continue;
-        const unsigned b = offs[d][cl], e = offs[d][cl + 1];
-        if (e <= b) continue;
-        if (nr > 0 && rb[nr - 1] + (rpre[nr] - rpre[nr - 1]) == b) {
-          rpre[nr] += e - b;   // extends the previous run
-        } else {
-          rb[nr] = b;
-          rpre[nr] = total;
-          rpre[nr + 1] = total + (e - b);
-          ++nr;
-        }
-        total = rpre[nr];
-      }
+                                                   unsigned* rb, unsigned* rpre, int* nranges) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const int d = lane >> 3, cl = lane & 7;
+    auto src = [&](int dd, int* Sx, int* Sy, int* Sz) {
+      *Sx = (Tx - ((dd >> 2) & 1) + g.ntx) % g.ntx;
+      *Sy = (Ty - ((dd >> 1) & 1) + g.nty) % g.nty;
+      *Sz = (Tz - (dd & 1) + g.ntz) % g.ntz;
+      return (*Sx * g.nty + *Sy) * g.ntz + *Sz;
+    };
+    int Sx, Sy, Sz;
+    const int S = src(d, &Sx, &Sy, &Sz);
+    bool dup = false;   // a periodic wrap can make two offsets name one tile
+    for (int e = 0; e < d; ++e) {
+      int ex, ey, ez;
+      dup |= src(e, &ex, &ey, &ez) == S;
     }
-    if (nr == 0) rpre[0] = 0;
-    *nranges = nr;
+    const unsigned need = (Sx != Tx ? 4u : 0u) | (Sy != Ty ? 2u : 0u) | (Sz != Tz ? 1u : 0u);
+    unsigned b = 0, len = 0;
+    if (!dup && ((unsigned)cl & need) == need) {
+      b = offsets[(unsigned)S * 8u + cl];
+      const unsigned e = offsets[(unsigned)S * 8u + cl + 1];
+      len = e > b ? e - b : 0u;
+    }
+    unsigned incl = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    const uint64_t nonempty = __ballot(len > 0);
+    const int k = __popcll(nonempty & ((1ull << lane) - 1ull));
+    if (len > 0) {
+      rb[k] = b;
+      rpre[k] = incl - len;
+    }
+    if (lane == 63) {
+      *nranges = __popcll(nonempty);
+      rpre[__popcll(nonempty)] = incl;
+    }
   }
   __syncthreads();
 }
@@ -338,10 +338,9 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
   grid += moff;
 
-  __shared__ unsigned offs[8][9];
-  __shared__ unsigned rb[16], rpre[17];
+  __shared__ unsigned rb[64], rpre[65];
   __shared__ int nranges;
-  tile_source_ranges(offsets, g, Tx, Ty, Tz, offs, rb, rpre, &nranges);
+  tile_source_ranges(offsets, g, Tx, Ty, Tz, rb, rpre, &nranges);
   const int nr = nranges;
   const unsigned total = rpre[nr];
   int r = 0;
@@ -555,10 +554,9 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
 #pragma unroll
   for (int a = 0; a < 3; ++a) sc[a] = value_scale(amax[a]);
 
-  __shared__ unsigned offs[8][9];
-  __shared__ unsigned rb[16], rpre[17];
+  __shared__ unsigned rb[64], rpre[65];
   __shared__ int nranges;
-  tile_source_ranges(offsets, g, Tx, Ty, Tz, offs, rb, rpre, &nranges);
+  tile_source_ranges(offsets, g, Tx, Ty, Tz, rb, rpre, &nranges);
   const int nr = nranges;
   const unsigned total = rpre[nr];
   int r = 0;
