@@ -288,7 +288,6 @@ float lpt_offset = 0.5f;   // py/lpt.cpp:29
 }  // namespace
 
 void free_paint_scratch(PaintScratch& s) {
-  dfree(s.keys);
   dfree(s.counts);
   dfree(s.offsets);
   dfree(s.cursor);
@@ -299,11 +298,10 @@ void free_paint_scratch(PaintScratch& s) {
 }
 
 void ensure_paint_scratch(PaintScratch& s, size_t np, size_t nbins) {
-  if (s.keys && s.np >= np && s.nbins == nbins) return;
+  if (s.rec && s.np >= np && s.nbins == nbins) return;
   if (g_inited) FS2_HIP(hipStreamSynchronize(g_stream));
   free_paint_scratch(s);
   const size_t cap = std::max<size_t>(np, 1);
-  s.keys = (unsigned*)dmalloc(cap * sizeof(unsigned));
   s.counts = (unsigned*)dmalloc(nbins * sizeof(unsigned));
   s.offsets = (unsigned*)dmalloc((nbins + 1) * sizeof(unsigned));
   s.cursor = (unsigned*)dmalloc(nbins * sizeof(unsigned));

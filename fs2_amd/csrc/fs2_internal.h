@@ -158,7 +158,6 @@ double grid_sum(const float* grid, int n);   // FP64 sum over the n^3 cells
 // Tiled paint (paint_tiled.hip): device scratch sized for np particles and
 // nbins = paint_tiled_bins(n) bins.
 struct PaintScratch {
-  unsigned* keys = nullptr;      // [np]
   unsigned* counts = nullptr;    // [nbins]
   unsigned* offsets = nullptr;   // [nbins + 1]
   unsigned* cursor = nullptr;    // [nbins]

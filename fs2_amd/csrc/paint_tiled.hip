@@ -142,8 +142,23 @@ __global__ void k_fill_u32(unsigned* __restrict__ a, int64_t n, unsigned v) {
   if (i < n) a[i] = v;
 }
 
+// Bin of a (wrapped) position: tile of its lower CIC cell and spill class,
+// offset by the box's bin range in batched launches.
+__device__ __forceinline__ unsigned bin_key(const TileGeom& g, float px, float py, float pz,
+                                            int64_t i) {
+  const Cic1 cx = cic1(px, g.dx_inv, g.n);
+  const Cic1 cy = cic1(py, g.dx_inv, g.n);
+  const Cic1 cz = cic1(pz, g.dx_inv, g.n);
+  unsigned key = key_of(g, cx.i0, cy.i0, cz.i0);
+  if (g.nbox > 1) key += ((unsigned)i / g.npb) * g.bins_box;
+  return key;
+}
+
+// K1: wrap (written back only where it changes x) and bin counts.  The
+// scatter recomputes the same key from the written positions, so no key
+// array is stored.
 __global__ void k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L, int do_wrap,
-                      TileGeom g, unsigned* __restrict__ keys, unsigned* __restrict__ counts) {
+                      TileGeom g, unsigned* __restrict__ counts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < np;
   unsigned key = kNoKey;
@@ -158,12 +173,7 @@ __global__ void k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L
         p[a] = w;
       }
     }
-    const Cic1 cx = cic1(p[0], g.dx_inv, g.n);
-    const Cic1 cy = cic1(p[1], g.dx_inv, g.n);
-    const Cic1 cz = cic1(p[2], g.dx_inv, g.n);
-    key = key_of(g, cx.i0, cy.i0, cz.i0);
-    if (g.nbox > 1) key += ((unsigned)i / g.npb) * g.bins_box;
-    keys[i] = key;
+    key = bin_key(g, p[0], p[1], p[2], i);
   }
   wave_aggregated_count(counts, key, valid);
 }
@@ -249,16 +259,19 @@ __global__ void k_scan_add(const unsigned* partial, const unsigned* __restrict__
 // One 16-byte record per particle in bin order: position and the particle's
 // index (bits in .w), which the tiled readout uses to write forces back in
 // id order.
-__global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t np,
-                          const unsigned* __restrict__ keys, unsigned* __restrict__ cursor,
-                          float4* __restrict__ rec) {
+__global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t np, TileGeom g,
+                          unsigned* __restrict__ cursor, float4* __restrict__ rec) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < np;
-  const unsigned key = valid ? keys[i] : kNoKey;
-  const unsigned slot = wave_aggregated_inc(cursor, key, valid);
+  float p[3] = {0.f, 0.f, 0.f};
+  unsigned key = kNoKey;
   if (valid) {
-    rec[slot] = make_float4(x[i], x[i + stride], x[i + 2 * stride], __uint_as_float((unsigned)i));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = x[i + a * stride];
+    key = bin_key(g, p[0], p[1], p[2], i);
   }
+  const unsigned slot = wave_aggregated_inc(cursor, key, valid);
+  if (valid) rec[slot] = make_float4(p[0], p[1], p[2], __uint_as_float((unsigned)i));
 }
 
 // Source tiles S = T - d, d in {0,1}^3 (periodic).  A particle of S
@@ -749,8 +762,8 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
   launch("paint_clear_bins", (double)nbins * 4.0, k_fill_u32, dim3(blocks_for(nbins)),
          dim3(kBlock), 0, s.counts, nbins, 0u);
   if (p.np > 0) {
-    launch("paint_bin", (double)p.np * 16.0, k_bin, dim3(blocks_for(p.np)), dim3(kBlock), 0, p.x,
-           p.stride, p.np, boxsize, wrap ? 1 : 0, g, s.keys, s.counts);
+    launch("paint_bin", (double)p.np * 12.0, k_bin, dim3(blocks_for(p.np)), dim3(kBlock), 0, p.x,
+           p.stride, p.np, boxsize, wrap ? 1 : 0, g, s.counts);
   }
   const int nb = (int)((nbins + kScanChunk - 1) / kScanChunk);
   if (nb > kScanChunk) fail(FS2_ERR_VALUE, "too many paint bins (%lld)", (long long)nbins);
@@ -764,7 +777,7 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
          s.offsets, s.cursor);
   if (p.np > 0) {
     launch("paint_scatter", (double)p.np * 28.0, k_scatter, dim3(blocks_for(p.np)), dim3(kBlock),
-           0, (const float*)p.x, p.stride, p.np, (const unsigned*)s.keys, s.cursor, s.rec);
+           0, (const float*)p.x, p.stride, p.np, g, s.cursor, s.rec);
   }
   const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0 * g.nbox;
   // fixed-point unit 2^-k with fac * 2^k <= 2^30, so every term fits an int32
