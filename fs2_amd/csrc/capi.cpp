@@ -109,6 +109,17 @@ hipStream_t stream() {
   return g_stream;
 }
 
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    device_init();
+    int v = 0;
+    FS2_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, 0));
+    cus = v > 0 ? v : 1;
+  }
+  return cus;
+}
+
 hipStream_t library_stream() {
   if (!g_inited) device_init();
   return g_stream;

@@ -20,6 +20,7 @@
 // out[(j/Ns) Ns R + j mod Ns + r Ns].  Radices 2, 3, 4, 8, 16; twiddles come
 // from an FP64-computed table rounded once to FP32.  Inverse transforms use
 // IDFT(x) = conj(DFT(conj(x))).
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -634,6 +635,61 @@ k_fft_z_c2r(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
   }
 }
 
+// Persistent c2r z pass for large meshes: each block stages the M-point and
+// split twiddles in LDS once and then walks rows P at a time (grid-stride),
+// instead of every one-row block fetching ~20 twiddles from L2.
+
+template <int M, int P>
+__global__ void __launch_bounds__(Plan<M>::T * P)
+k_fft_z_c2r_p(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ twM,
+              const float2* __restrict__ twN) {
+  constexpr int T = Plan<M>::T;
+  using L = ContigLayout;
+  constexpr int LS = L::size(M);
+  __shared__ float2 s[P * LS];
+  __shared__ float2 stM[M];
+  __shared__ float2 stN[M];
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    stM[i] = twM[i];
+    stN[i] = twN[i];
+  }
+  const int tid = threadIdx.x;
+  for (int64_t row0 = (int64_t)blockIdx.x * P; row0 < nrows; row0 += (int64_t)gridDim.x * P) {
+    // opaque thread coordinates: keeps the compiler from holding every LDS
+    // address of the transform live across iterations
+    int t = tid % T, p = tid / T;
+    asm volatile("" : "+v"(t), "+v"(p));
+    float2* sp = s + p * LS;
+    const int64_t row = row0 + p;
+    const bool valid = row < nrows;
+    float2* r = data + row * cpitch;
+    __syncthreads();   // staging, and the previous row's LDS reads, are done
+#pragma unroll
+    for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = valid ? r[t + i * T] : make_float2(0.f, 0.f);
+    if (t == 0) sp[L::at(M)] = valid ? r[M] : make_float2(0.f, 0.f);
+    __syncthreads();
+    float2 z[M / T];
+#pragma unroll
+    for (int i = 0; i < M / T; ++i) {
+      const int k = t + i * T;
+      float2 A = sp[L::at(k)];
+      float2 B = sp[L::at(M - k)];
+      if (k == 0) {
+        A.y = 0.f;
+        B.y = 0.f;
+      }
+      const float2 Bc = cconj(B);
+      const float2 E = cadd(A, Bc);
+      const float2 O = cmul(csub(A, Bc), cconj(stN[k]));
+      z[i] = cconj(cadd(E, mul_posi(O)));
+    }
+    fft_regs<M, L>(z, sp, t, stM);
+#pragma unroll
+    for (int i = 0; i < M / T; ++i)
+      if (valid) r[t + i * T] = cconj(z[i]);
+  }
+}
+
 // ---- host side ---------------------------------------------------------------------------------
 const float2* twiddles(int N) {
   static std::mutex mu;
@@ -684,9 +740,25 @@ struct Impl {
            twiddles(N), g, bstride());
   }
 
+  // rows per block of the persistent c2r z pass (M >= 256) and its grid
+  static constexpr int PP = Plan<M>::T >= 256 ? 1 : 256 / Plan<M>::T;
+  // all blocks resident at once, so the grid-stride loop has no second wave
+  template <typename K>
+  static unsigned persistent_grid(K kernel, int64_t nrows) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+      int v = 0;
+      FS2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, Plan<M>::T * PP, 0));
+      per_cu = v > 0 ? v : 1;
+    }
+    const int64_t want = (nrows + PP - 1) / PP;
+    return (unsigned)std::min<int64_t>(want, (int64_t)device_cus() * per_cu);
+  }
+
   static void z_r2c(float* grid, int nb) {
     const int64_t nrows = (int64_t)n * n * nb;
     const double bytes = ((double)n * n * n * 4.0 + grid_bytes()) * nb;
+    // (a persistent r2c, like the c2r below, measured slower at 1024^3)
     launch("fft_z_r2c", bytes, k_fft_z_r2c<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
            dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
            twiddles(N));
@@ -695,9 +767,16 @@ struct Impl {
   static void z_c2r(float* grid, int nb) {
     const int64_t nrows = (int64_t)n * n * nb;
     const double bytes = ((double)n * n * n * 4.0 + grid_bytes()) * nb;
-    launch("fft_z_c2r", bytes, k_fft_z_c2r<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
-           dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
-           twiddles(N));
+    if constexpr (M >= 256) {
+      launch("fft_z_c2r", bytes, k_fft_z_c2r_p<M, PP>,
+             dim3(persistent_grid(k_fft_z_c2r_p<M, PP>, nrows)),
+             dim3(Plan<M>::T * PP), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
+             twiddles(N));
+    } else {
+      launch("fft_z_c2r", bytes, k_fft_z_c2r<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
+             dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
+             twiddles(N));
+    }
   }
 
   static void r2c(float* grid, int nb) {

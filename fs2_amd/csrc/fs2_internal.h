@@ -61,6 +61,7 @@ void device_init();                 // throws Error(FS2_ERR_DEVICE) without a GP
 // stream, or the stream of the innermost live StreamScope.
 hipStream_t stream();
 hipStream_t library_stream();
+int device_cus();                   // compute units of the device
 class StreamScope {
  public:
   explicit StreamScope(hipStream_t s);
