@@ -907,6 +907,49 @@ int fs2_cola_drift(fs2_particles* p, double a_pos) {
   });
 }
 
+// ---- standard leapfrog --------------------------------------------------------------------------
+int fs2_leapfrog_factors(double ai, double af, double* kick_factor, double* dt) {
+  return guard([&] {
+    if (kick_factor) *kick_factor = host::leapfrog_Sphi(ai, af);   // leapfrog.cpp:96-115
+    if (dt) *dt = host::leapfrog_Sq(ai, af);                          // leapfrog.cpp:117-141
+  });
+}
+
+int fs2_leapfrog_initial_velocity(fs2_particles* p, double a) {
+  return guard([&] {
+    check_p(p);
+    // leapfrog.cpp:22-28: Float da1, da2, Dv, D2v (D2v from da2 here)
+    const float da1 = (float)host::D_growth(a);
+    const float da2 = (float)host::D2_growth(a, (double)da1);
+    const float Dv = (float)host::Dv_growth(a, (double)da1);
+    const float D2v = (float)host::D2v_growth(a, (double)da2);
+    leapfrog_initial_velocity(p->v(), p->dx1(), p->dx2(), (int64_t)p->np_alloc,
+                              (int64_t)p->np_local, Dv, D2v);
+    p->a_v = a;
+  });
+}
+
+int fs2_leapfrog_kick(fs2_particles* p, double a_vel) {
+  return guard([&] {
+    check_p(p);
+    const double kf = host::leapfrog_Sphi(p->a_v, a_vel);
+    leapfrog_kick(p->v(), p->f(), (int64_t)p->np_alloc, (int64_t)p->np_local,
+                  -1.5 * host::omega_m(), (float)kf);   // kick_factor is a Float
+    p->a_v = a_vel;
+  });
+}
+
+int fs2_leapfrog_drift(fs2_particles* p, double a_pos) {
+  return guard([&] {
+    check_p(p);
+    const double dt = host::leapfrog_Sq(p->a_x, a_pos);
+    flush_wrap(p);
+    leapfrog_drift(p->x(), p->v(), (int64_t)p->np_alloc, (int64_t)p->np_local, dt);
+    touch_x(p);
+    p->a_x = a_pos;
+  });
+}
+
 int fs2_cola_velocity(fs2_particles* p, float* out) {
   return guard([&] {
     check_p(p);

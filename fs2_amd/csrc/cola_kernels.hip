@@ -1,4 +1,5 @@
-// COLA kick / drift kernels for gfx950 (py/cola.cpp:48-148).
+// COLA kick / drift kernels for gfx950 (py/cola.cpp:48-148) and the
+// standard leapfrog ones (py/leapfrog.cpp:18-94).
 //
 // The FP64 time coefficients come from the host; the per-particle update
 // keeps the reference's mixed-precision expression order, so contraction is
@@ -66,7 +67,64 @@ __global__ void k_scale(float* __restrict__ out, const float* __restrict__ in, i
   for (int a = 0; a < 3; ++a) out[i + a * stride] = s * in[i + a * stride];
 }
 
+// leapfrog.cpp:31-33: v = dx1 Dv + dx2 D2v (Float)
+__global__ void k_lf_velocity(float* __restrict__ v, const float* __restrict__ dx1,
+                              const float* __restrict__ dx2, int64_t stride, int64_t np, float Dv,
+                              float D2v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t j = i + a * stride;
+    v[j] = dx1[j] * Dv + dx2[j] * D2v;
+  }
+}
+
+// leapfrog.cpp:61-65: v += -1.5*om*f*kick_factor, the product in double
+// (om is double) and rounded once into v.
+__global__ void k_lf_kick(float* __restrict__ v, const float* __restrict__ f, int64_t stride,
+                          int64_t np, double m15om, float kf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t j = i + a * stride;
+    v[j] = (float)((double)v[j] + m15om * (double)f[j] * (double)kf);
+  }
+}
+
+// leapfrog.cpp:86-90: x += v*dt with dt double
+__global__ void k_lf_drift(float* __restrict__ x, const float* __restrict__ v, int64_t stride,
+                           int64_t np, double dt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t j = i + a * stride;
+    x[j] = (float)((double)x[j] + (double)v[j] * dt);
+  }
+}
+
 }  // namespace
+
+void leapfrog_initial_velocity(float* v, const float* dx1, const float* dx2, int64_t stride,
+                               int64_t np, float Dv, float D2v) {
+  if (np == 0) return;
+  launch("leapfrog_velocity", (double)np * 36.0, k_lf_velocity, dim3(blocks_for(np)),
+         dim3(kBlock), 0, v, dx1, dx2, stride, np, Dv, D2v);
+}
+
+void leapfrog_kick(float* v, const float* f, int64_t stride, int64_t np, double m15om, float kf) {
+  if (np == 0) return;
+  launch("leapfrog_kick", (double)np * 36.0, k_lf_kick, dim3(blocks_for(np)), dim3(kBlock), 0,
+         v, f, stride, np, m15om, kf);
+}
+
+void leapfrog_drift(float* x, const float* v, int64_t stride, int64_t np, double dt) {
+  if (np == 0) return;
+  launch("leapfrog_drift", (double)np * 36.0, k_lf_drift, dim3(blocks_for(np)), dim3(kBlock), 0,
+         x, v, stride, np, dt);
+}
 
 void cola_kick(float* v, const float* f, const float* dx1, const float* dx2, int64_t stride,
                int64_t np, double m15om, float kf, float q1, float q2) {

@@ -209,6 +209,11 @@ void cola_drift(float* x, const float* v, const float* dx1, const float* dx2,
 void cola_velocity(float* out, const float* v, const float* dx1, const float* dx2,
                    int64_t stride, int64_t np, float Dv, float D2v);
 void scale_field(float* out, const float* in, int64_t stride, int64_t np, float s);
+// standard (non-COLA) leapfrog, py/leapfrog.cpp:18-94
+void leapfrog_initial_velocity(float* v, const float* dx1, const float* dx2, int64_t stride,
+                               int64_t np, float Dv, float D2v);
+void leapfrog_kick(float* v, const float* f, int64_t stride, int64_t np, double m15om, float kf);
+void leapfrog_drift(float* x, const float* v, int64_t stride, int64_t np, double dt);
 
 // ---- LPT kernels (lpt_kernels.hip) --------------------------------------------------
 void white_noise(float* grid, int n, uint64_t seed);
@@ -270,6 +275,8 @@ double hubble(double a);
 double Dv_growth(double a, double D);
 double D2v_growth(double a, double D2);
 double drift_Sq(double ai, double af, double av);
+double leapfrog_Sphi(double ai, double af);
+double leapfrog_Sq(double ai, double af);
 }  // namespace host
 
 }  // namespace fs2

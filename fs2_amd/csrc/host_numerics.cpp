@@ -153,6 +153,24 @@ double Dv_growth(double a, double D) {
 
 double D2v_growth(double a, double D2) { return 2.0 * a * a * D2 * hubble(a) * f_growth_rate(a); }
 
+// py/leapfrog.cpp:96-115: SphiStd = int_ai^af da / (a^2 H(a))
+double leapfrog_Sphi(double ai, double af) {
+  check_init();
+  if (!(ai > 0.0)) fail(FS2_ERR_VALUE, "leapfrog kick from a = %g", ai);
+  const double om = g_omega_m;
+  auto f = [om](double a) { return 1.0 / (std::sqrt(om / (a * a * a) + 1.0 - om) * a * a); };
+  return integrate(f, ai, af);
+}
+
+// py/leapfrog.cpp:117-141: SqStd = int_ai^af da / (a^3 H(a))
+double leapfrog_Sq(double ai, double af) {
+  check_init();
+  if (!(ai > 0.0)) fail(FS2_ERR_VALUE, "leapfrog drift from a = %g", ai);
+  const double om = g_omega_m;
+  auto f = [om](double a) { return 1.0 / (std::sqrt(om / (a * a * a) + 1.0 - om) * a * a * a); };
+  return integrate(f, ai, af);
+}
+
 // py/cola.cpp:152-179: int_ai^af a^n / (a^3 H(a)) da / av^n, n = -2.5
 double drift_Sq(double ai, double af, double av) {
   check_init();

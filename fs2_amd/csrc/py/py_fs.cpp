@@ -759,6 +759,45 @@ PyObject* py_cola_velocity(PyObject*, PyObject* args) {
   return arr;
 }
 
+// ---------------------------------------------------------------- leapfrog
+// py/py_leapfrog.cpp:4-66
+PyObject* py_leapfrog_initial_velocity(PyObject*, PyObject* args) {
+  PyObject* o;
+  double a;
+  if (!PyArg_ParseTuple(args, "Od", &o, &a)) return nullptr;
+  fs2_particles* p = as_particles(o);
+  PY_ASSERT_PTR(p);
+  CALL(fs2_leapfrog_initial_velocity(p, a));
+  Py_RETURN_NONE;
+}
+
+PyObject* py_leapfrog_kick(PyObject*, PyObject* args) {
+  PyObject* o;
+  double a_vel;
+  if (!PyArg_ParseTuple(args, "Od", &o, &a_vel)) return nullptr;
+  fs2_particles* p = as_particles(o);
+  PY_ASSERT_PTR(p);
+  CALL(fs2_leapfrog_kick(p, a_vel));
+  Py_RETURN_NONE;
+}
+
+PyObject* py_leapfrog_drift(PyObject*, PyObject* args) {
+  PyObject* o;
+  double a_pos;
+  if (!PyArg_ParseTuple(args, "Od", &o, &a_pos)) return nullptr;
+  fs2_particles* p = as_particles(o);
+  PY_ASSERT_PTR(p);
+  CALL(fs2_leapfrog_drift(p, a_pos));
+  Py_RETURN_NONE;
+}
+
+PyObject* py_leapfrog_factors(PyObject*, PyObject* args) {
+  double ai, af, kf, dt;
+  if (!PyArg_ParseTuple(args, "dd", &ai, &af)) return nullptr;
+  CALL(fs2_leapfrog_factors(ai, af, &kf, &dt));
+  return Py_BuildValue("dd", kf, dt);
+}
+
 // ---------------------------------------------------------------- fft
 PyObject* py_fft_alloc(PyObject*, PyObject* args) {
   int nc;
@@ -968,6 +1007,11 @@ PyMethodDef methods[] = {
     M("_pm_force_mesh", py_pm_force_mesh, "new: real force mesh of one axis"),
     M("_pm_power_spectrum", py_pm_power_spectrum, "new: (k, P, nmodes) of the density"),
     M("_cola_kick", py_cola_kick, "_cola_kick(_particles, a_vel)"),
+    M("_leapfrog_initial_velocity", py_leapfrog_initial_velocity,
+      "_leapfrog_initial_velocity(_particles, a)"),
+    M("_leapfrog_kick", py_leapfrog_kick, "_leapfrog_kick(_particles, a_vel)"),
+    M("_leapfrog_drift", py_leapfrog_drift, "_leapfrog_drift(_particles, a_pos)"),
+    M("_leapfrog_factors", py_leapfrog_factors, "new: (kick_factor, dt) of SphiStd/SqStd"),
     M("_cola_drift", py_cola_drift, "_cola_drift(_particles, a_pos)"),
     M("_cola_velocity", py_cola_velocity, "new: physical COLA velocity"),
     M("_fft_alloc", py_fft_alloc, "_fft_alloc(nc)"),

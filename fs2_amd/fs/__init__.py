@@ -2,7 +2,7 @@
 
 Mirrors py/fs/__init__.py of the reference: importing the package
 initialises communication (one rank here) and exposes the same modules.
-Leapfrog, FoF and the kd-tree are not part of this package.
+FoF and the kd-tree are not part of this package.
 """
 import atexit
 
@@ -12,6 +12,7 @@ from . import cosmology
 from . import lpt
 from . import pm
 from . import cola
+from . import leapfrog
 from . import msg
 from .particles import Particles
 from .power import PowerSpectrum

@@ -226,6 +226,20 @@ int fs2_cola_drift(fs2_particles* p, double a_pos);
 /* py/cola.cpp:129-148 cola_velocity: (np_local, 3) FP32 into out */
 int fs2_cola_velocity(fs2_particles* p, float* out);
 
+/* ---- standard leapfrog ----------------------------------------------------- */
+/* py/leapfrog.cpp:96-141 SphiStd(ai, af) and SqStd(ai, af): the kick factor
+ * int da/(a^2 H) and the drift factor int da/(a^3 H), FP64. */
+int fs2_leapfrog_factors(double ai, double af, double* kick_factor, double* dt);
+/* py/leapfrog.cpp:18-38 / py_leapfrog.cpp:4-22 _leapfrog_initial_velocity:
+ * v = Dv(a) dx1 + D2v(a) dx2, a_v = a */
+int fs2_leapfrog_initial_velocity(fs2_particles* p, double a);
+/* py/leapfrog.cpp:40-68 / py_leapfrog.cpp:24-44 _leapfrog_kick:
+ * v += -1.5 Om f SphiStd(a_v, a_vel) */
+int fs2_leapfrog_kick(fs2_particles* p, double a_vel);
+/* py/leapfrog.cpp:70-94 / py_leapfrog.cpp:46-66 _leapfrog_drift:
+ * x += v SqStd(a_x, a_pos) */
+int fs2_leapfrog_drift(fs2_particles* p, double a_pos);
+
 /* ---- 2LPT ---------------------------------------------------------------- */
 /* py/py_lpt.cpp:9-42 _lpt(nc, boxsize, a, seed, ps, kind) with kind one of
  * "zeldovich", "2lpt", "cola".  The Gaussian field is seeded white noise

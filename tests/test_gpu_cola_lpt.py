@@ -100,6 +100,60 @@ def test_kick_drift_match_oracle(fs, cosmo):
     assert p.times[0] == 0.3
 
 
+def test_leapfrog_kernels_match_oracle(fs, cosmo):
+    """py/leapfrog.cpp:18-94: initial velocity, kick, drift."""
+    from oracle import leapfrog as olf
+    _announce("leapfrog kernels")
+    rng = np.random.default_rng(4)
+    N, L = 1000, 64.0
+    p = fs.Particles(N, L)
+    p.append(rng.uniform(0, L, (N, 3)))
+    for name in ("dx1", "dx2", "force"):
+        p.set_field(name, rng.standard_normal((N, 3)).astype(np.float32))
+    dx1, dx2, f = p.dx1.astype(np.float64), p.dx2.astype(np.float64), p.force.astype(np.float64)
+    fs.leapfrog.set_initial_velocity(p, 0.1)
+    v0 = olf.initial_velocity(dx1, dx2, cosmo, 0.1)
+    assert np.abs(p.v - v0).max() < 1e-6 * np.abs(v0).max()
+    assert p.times[1] == 0.1
+    p.set_times(0.1, 0.1, 0.1)
+    x0, v0 = p.x.astype(np.float64), p.v.astype(np.float64)
+    fs.leapfrog.kick(p, 0.15)
+    v1 = olf.kick(v0, f, cosmo, 0.1, 0.15)
+    assert np.abs(p.v - v1).max() < 1e-6 * (1 + np.abs(v1).max())
+    fs.leapfrog.drift(p, 0.2)
+    x1 = olf.drift(x0, p.v.astype(np.float64), cosmo, 0.1, 0.2)
+    assert np.abs(p.x - x1).max() < 4e-6 * L
+    assert p.times[:2] == (0.2, 0.15)
+
+
+def test_leapfrog_pm_run_matches_oracle(fs, cosmo, eh):
+    """5 standard leapfrog steps with PM forces from 2LPT ICs (v from
+    set_initial_velocity): positions within 1e-3 mesh cells of FP64."""
+    from oracle import leapfrog as olf
+    _announce("leapfrog run 16^3/32^3")
+    n_p, nc, L, a0, a1, nstep = 16, 32, 64.0, 0.1, 0.5, 5
+    p = fs.lpt.init(n_p, L, a0, eh, 13, "2lpt")
+    fs.leapfrog.set_initial_velocity(p, a0)
+    dx1, dx2 = p.dx1.astype(np.float64), p.dx2.astype(np.float64)
+    x = p.x.astype(np.float64)
+    v = olf.initial_velocity(dx1, dx2, cosmo, a0)
+    fs.pm.init(nc, nc / n_p, L)
+    a_x = a_v = a0
+    for a_vel, a_pos in ocola.schedule(a0, a1, nstep):
+        fs.pm.force(p)
+        fs.leapfrog.kick(p, a_vel)
+        fs.leapfrog.drift(p, a_pos)
+        xw = np.mod(x, L)
+        f = opm.force(xw, nc, L, nc / n_p)
+        v = olf.kick(v, f, cosmo, a_v, a_vel)
+        a_v = a_vel
+        x = olf.drift(xw, v, cosmo, a_x, a_pos)
+        a_x = a_pos
+    d = p.x.astype(np.float64) - x
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() / (L / nc) < 1e-3, np.abs(d).max() / (L / nc)
+
+
 def test_cola_run_matches_oracle(fs, cosmo, eh):
     """10 COLA steps from 2LPT ICs: final positions within 1e-3 mesh cells."""
     _announce("cola run 16^3/32^3")

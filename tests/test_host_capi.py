@@ -48,6 +48,7 @@ def test_extension_method_table_matches_reference_names(fs):
         "_pm_compute_force", "_pm_compute_density", "_pm_domain_init", "_pm_send_positions",
         "_pm_check_total_density", "_pm_get_forces", "_pm_set_packet_size", "_cola_kick",
         "_cola_drift", "_fft_alloc", "_fft_set_test_data", "_fft_fx_global_as_array",
+        "_leapfrog_initial_velocity", "_leapfrog_kick", "_leapfrog_drift",
         "config_precision"]
     missing = [n for n in expected if not hasattr(fs._fs, n)]
     assert not missing, missing
@@ -81,6 +82,17 @@ def test_host_cola_coefficients_match_oracle(fs):
         assert abs(dt - rdt) < 1e-12 * abs(rdt)
         assert abs(da1 - rda1) < 1e-11 * abs(rda1)
         assert abs(da2 - rda2) < 1e-11 * abs(rda2)
+
+
+def test_host_leapfrog_factors_match_oracle(fs):
+    from oracle import leapfrog as olf
+    om = 0.31
+    fs.cosmology.init(om)
+    c = ocosmo.Cosmology(om)
+    for ai, af in [(0.1, 0.145), (0.5, 0.6), (0.955, 1.0)]:
+        kf, dt = fs.leapfrog.factors(ai, af)
+        assert abs(kf - olf.kick_factor(c, ai, af)) < 1e-12 * kf
+        assert abs(dt - olf.drift_factor(c, ai, af)) < 1e-12 * dt
 
 
 def test_table_power_spectrum_matches_oracle(fs):

@@ -69,6 +69,25 @@ def test_cola_coefficients_eds_closed_form():
     assert abs(dt - exact) < 1e-12 * abs(exact)
 
 
+# ---- standard leapfrog factors
+class _Om:
+    def __init__(self, om):
+        self.om = om
+
+
+@pytest.mark.parametrize("ai,af", [(0.1, 0.2), (0.3, 1.0)])
+def test_leapfrog_factors_closed_forms(ai, af):
+    """py/leapfrog.cpp:96-141 integrals: Om = 1 (EdS, H = a^-3/2) gives
+    Sphi = 2(sqrt af - sqrt ai), Sq = 2(ai^-1/2 - af^-1/2); Om = 0 (de
+    Sitter, H = 1) gives Sphi = 1/ai - 1/af, Sq = (ai^-2 - af^-2)/2."""
+    from oracle import leapfrog as olf
+    eds, ds = _Om(1.0), _Om(0.0)
+    assert olf.kick_factor(eds, ai, af) == pytest.approx(2 * (af**0.5 - ai**0.5), rel=1e-12)
+    assert olf.drift_factor(eds, ai, af) == pytest.approx(2 * (ai**-0.5 - af**-0.5), rel=1e-12)
+    assert olf.kick_factor(ds, ai, af) == pytest.approx(1 / ai - 1 / af, rel=1e-12)
+    assert olf.drift_factor(ds, ai, af) == pytest.approx((ai**-2 - af**-2) / 2, rel=1e-12)
+
+
 # ---- CIC: test/test_pm_cic.py known answers and closed-form weights
 @pytest.mark.parametrize("xc", [0, 1, 1.5, 3, 3.5, 4])
 def test_cic_one_particle_known_answer(xc):
