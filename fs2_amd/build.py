@@ -21,6 +21,8 @@ CSRC = os.path.join(HERE, 'csrc')
 INCLUDE = os.path.join(ROOT, 'include')
 BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libfs2hip.so')
+IOLIB = os.path.join(HERE, 'libfs2io.so')
+HDF5_DIR = os.environ.get('HDF5_DIR', '/opt/conda')
 PYEXT = os.path.join(HERE, 'fs', '_fs.so')
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
@@ -63,6 +65,24 @@ def build_lib(jobs=8):
     return LIB
 
 
+def build_iolib():
+    """libfs2io.so (HDF5 output) where an HDF5 C library is present; returns
+    its path or None."""
+    inc, lib = os.path.join(HDF5_DIR, 'include'), os.path.join(HDF5_DIR, 'lib')
+    if not (os.path.exists(os.path.join(inc, 'hdf5.h')) and
+            os.path.exists(os.path.join(lib, 'libhdf5.so'))):
+        return None
+    os.makedirs(BUILD, exist_ok=True)
+    obj = os.path.join(BUILD, 'io_hdf5.cpp.o')
+    _run([HIPCC, '-x', 'hip'] + HIP_FLAGS + ['-I', inc, '-c', os.path.join(CSRC, 'io_hdf5.cpp'),
+                                             '-o', obj])
+    tmp = IOLIB + '.tmp'
+    _run([HIPCC, f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', tmp, obj, '-L', HERE,
+          '-lfs2hip', '-L', lib, '-lhdf5', "-Wl,-rpath,$ORIGIN", '-Wl,-rpath,' + lib])
+    os.replace(tmp, IOLIB)
+    return IOLIB
+
+
 def build_pyext():
     import numpy
     py_inc = sysconfig.get_paths()['include']
@@ -70,13 +90,14 @@ def build_pyext():
     tmp = PYEXT + '.tmp'
     _run(['g++', '-O2', '-std=c++17', '-fPIC', '-shared', '-Wall',
           '-I', py_inc, '-I', numpy.get_include(), '-I', INCLUDE,
-          src, '-o', tmp, '-L', HERE, '-lfs2hip', "-Wl,-rpath,$ORIGIN/.."])
+          src, '-o', tmp, '-L', HERE, '-lfs2hip', '-ldl', "-Wl,-rpath,$ORIGIN/.."])
     os.replace(tmp, PYEXT)
     return PYEXT
 
 
 def build_all(jobs=8):
     build_lib(jobs)
+    build_iolib()
     build_pyext()
 
 

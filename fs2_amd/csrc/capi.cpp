@@ -907,6 +907,103 @@ int fs2_cola_drift(fs2_particles* p, double a_pos) {
   });
 }
 
+// ---- output: Gadget-1 binary ---------------------------------------------------------------
+namespace {
+// py/gadget_file.h:6-25: the 256-byte Gadget-1 header (layout kept for the format)
+struct GadgetHeader {
+  int32_t np[6];
+  double mass[6];
+  double time;
+  double redshift;
+  int32_t flag_sfr;
+  int32_t flag_feedback;
+  uint32_t np_total[6];
+  int32_t flag_cooling;
+  int32_t num_files;
+  double boxsize;
+  double omega0;
+  double omega_lambda;
+  double hubble_param;
+  int32_t flag_stellarage;
+  int32_t flag_metals;
+  uint32_t np_total_highword[6];
+  int32_t flag_entropy_instead_u;
+  char fill[60];
+};
+static_assert(sizeof(GadgetHeader) == 256, "Gadget-1 header is 256 bytes");
+
+// py/const.h:12-23: internal units 1/h Mpc, km/s, M_sun; rho_crit_0 = 3 H0^2/(8 pi G)
+double rho_crit_0() {
+  const double unit_length = 3.085678e24, unit_mass = 1.989e33, unit_velocity = 1.0e5;
+  const double G_cgs = 6.672e-8;
+  const double unit_time = unit_length / unit_velocity;
+  const double G = G_cgs * unit_mass * (unit_time * unit_time) /
+                   (unit_length * unit_length * unit_length);
+  const double H0 = 100.0;
+  return 3.0 * H0 * H0 / (8.0 * M_PI * G);
+}
+
+void put_block(std::FILE* fp, const void* data, size_t bytes, const char* path) {
+  const int32_t len = (int32_t)bytes;
+  if (std::fwrite(&len, 4, 1, fp) != 1 || (bytes && std::fwrite(data, 1, bytes, fp) != bytes) ||
+      std::fwrite(&len, 4, 1, fp) != 1)
+    fail(FS2_ERR_IO, "write to %s failed", path);
+}
+}  // namespace
+
+int fs2_cosmology_omega_m(double* out) {
+  return guard([&] { *out = host::omega_m(); });
+}
+
+int fs2_write_gadget(fs2_particles* p, const char* filename, int use_long_id, double h) {
+  return guard([&] {
+    check_p(p);
+    if (!filename) fail(FS2_ERR_VALUE, "null file name");
+    const uint64_t np = p->np_local;
+    // positions, velocities / sqrt(a) (Gadget convention), ids (gadget_file.cpp:70-114)
+    std::vector<float> x(3 * np), v(3 * np);
+    std::vector<uint64_t> id(np);
+    int rc = fs2_particles_get_field(p, FS2_FIELD_X, x.data());
+    if (!rc) rc = fs2_particles_get_field(p, FS2_FIELD_V, v.data());
+    if (!rc) rc = fs2_particles_get_id(p, id.data());
+    if (rc) fail(rc, "%s", fs2_last_error());
+    GadgetHeader hd;
+    std::memset(&hd, 0, sizeof(hd));
+    const double om = host::omega_m();
+    const uint64_t ntot = np;   // comm_sum of np_local over the one rank
+    hd.np[1] = (int32_t)np;
+    hd.mass[1] = om * rho_crit_0() * std::pow(p->boxsize, 3.0) / (double)ntot;
+    hd.time = p->a_x;
+    hd.redshift = 1.0 / hd.time - 1.0;
+    hd.np_total[1] = (uint32_t)ntot;
+    hd.np_total_highword[1] = (uint32_t)(ntot >> 32);
+    hd.num_files = 1;
+    hd.boxsize = p->boxsize;
+    hd.omega0 = om;
+    hd.omega_lambda = 1.0 - om;
+    hd.hubble_param = h;
+    const float vfac = (float)(1.0 / std::sqrt(p->a_x));
+    for (float& c : v) c = (float)(vfac * c);
+    std::FILE* fp = std::fopen(filename, "wb");
+    if (!fp) fail(FS2_ERR_IO, "unable to write to file: %s", filename);
+    try {
+      put_block(fp, &hd, sizeof(hd), filename);
+      put_block(fp, x.data(), x.size() * sizeof(float), filename);
+      put_block(fp, v.data(), v.size() * sizeof(float), filename);
+      if (use_long_id) {
+        put_block(fp, id.data(), id.size() * sizeof(uint64_t), filename);
+      } else {
+        std::vector<uint32_t> id32(id.begin(), id.end());
+        put_block(fp, id32.data(), id32.size() * sizeof(uint32_t), filename);
+      }
+    } catch (...) {
+      std::fclose(fp);
+      throw;
+    }
+    if (std::fclose(fp) != 0) fail(FS2_ERR_IO, "write to %s failed", filename);
+  });
+}
+
 // ---- standard leapfrog --------------------------------------------------------------------------
 int fs2_leapfrog_factors(double ai, double af, double* kick_factor, double* dt) {
   return guard([&] {

@@ -16,6 +16,8 @@
 #define NPY_NO_DEPRECATED_API NPY_1_7_API_VERSION
 #include <numpy/arrayobject.h>
 
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -759,6 +761,79 @@ PyObject* py_cola_velocity(PyObject*, PyObject* args) {
   return arr;
 }
 
+// ---------------------------------------------------------------- output
+// py/py_write.cpp: _write_gadget_binary(_particles, filename, use_long_id)
+PyObject* py_write_gadget_binary(PyObject*, PyObject* args) {
+  PyObject *o, *bytes, *longid;
+  if (!PyArg_ParseTuple(args, "OO&O", &o, PyUnicode_FSConverter, &bytes, &longid)) return nullptr;
+  fs2_particles* p = as_particles(o);
+  if (!p) {
+    Py_DECREF(bytes);
+    PyErr_SetString(PyExc_TypeError, "py_assersion error");
+    return nullptr;
+  }
+  const int use_long_id = PyObject_IsTrue(longid);
+  const int rc = use_long_id < 0 ? -1 : fs2_write_gadget(p, PyBytes_AsString(bytes), use_long_id, 1.0);
+  Py_DECREF(bytes);
+  if (rc < 0) return nullptr;
+  if (rc) return raise(rc);
+  Py_RETURN_NONE;
+}
+
+// The HDF5 writer is in libfs2io.so next to libfs2hip.so, opened on first
+// use so that the module loads where no HDF5 library is installed.
+typedef int (*write_hdf5_fn)(fs2_particles*, const char*, const char*);
+write_hdf5_fn hdf5_writer(std::string* why) {
+  static write_hdf5_fn fn = nullptr;
+  static std::string err;
+  if (fn || !err.empty()) {
+    *why = err;
+    return fn;
+  }
+  Dl_info info;
+  std::string dir = ".";
+  if (dladdr((void*)&hdf5_writer, &info) && info.dli_fname) {
+    dir = info.dli_fname;
+    dir = dir.substr(0, dir.find_last_of('/'));
+  }
+  const std::string path = dir + "/../libfs2io.so";
+  void* h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+  if (!h) {
+    const char* e = dlerror();
+    err = std::string("HDF5 output is not available (") + (e ? e : path) + ")";
+  } else {
+    fn = (write_hdf5_fn)dlsym(h, "fs2_write_hdf5");
+    if (!fn) err = "libfs2io.so does not export fs2_write_hdf5";
+  }
+  *why = err;
+  return fn;
+}
+
+// py/py_hdf5_io.cpp: _hdf5_write_particles(_particles, filename, var)
+PyObject* py_hdf5_write_particles(PyObject*, PyObject* args) {
+  PyObject *o, *bytes, *vbytes;
+  if (!PyArg_ParseTuple(args, "OO&O&", &o, PyUnicode_FSConverter, &bytes, PyUnicode_FSConverter,
+                        &vbytes))
+    return nullptr;
+  fs2_particles* p = as_particles(o);
+  std::string why;
+  write_hdf5_fn fn = p ? hdf5_writer(&why) : nullptr;
+  int rc = 0;
+  if (p && fn) rc = fn(p, PyBytes_AsString(bytes), PyBytes_AsString(vbytes));
+  Py_DECREF(bytes);
+  Py_DECREF(vbytes);
+  if (!p) {
+    PyErr_SetString(PyExc_TypeError, "py_assersion error");
+    return nullptr;
+  }
+  if (!fn) {
+    PyErr_SetString(PyExc_IOError, why.c_str());
+    return nullptr;
+  }
+  if (rc) return raise(rc);
+  Py_RETURN_NONE;
+}
+
 // ---------------------------------------------------------------- leapfrog
 // py/py_leapfrog.cpp:4-66
 PyObject* py_leapfrog_initial_velocity(PyObject*, PyObject* args) {
@@ -983,6 +1058,10 @@ PyMethodDef methods[] = {
     M("_particles_v_asarray", py_particles_v_asarray, "new: velocities (np_local, 3)"),
     M("_particles_set_field", py_particles_set_field, "new: upload one (np_local,3) field"),
     M("_particles_periodic_wrapup", py_particles_periodic_wrapup, "periodic wrap of x"),
+    M("_write_gadget_binary", py_write_gadget_binary,
+      "_write_gadget_binary(_particles, filename, use_long_id)"),
+    M("_hdf5_write_particles", py_hdf5_write_particles,
+      "_hdf5_write_particles(_particles, filename, var)"),
     M("_particles_clear", py_particles_clear, "_particles_clear(_particles)"),
     M("_particles_append", py_particles_append, "_particles_append(_particles, x)"),
     M("_particles_times", py_particles_times, "new: (a_x, a_v, a_f)"),

@@ -38,6 +38,31 @@ class Particles(object):
     def periodic_wrapup(self):
         c._particles_periodic_wrapup(self._particles)
 
+    def save_gadget_binary(self, filename, use_longid=False):
+        """Gadget-1 binary snapshot (py/fs/particles.py:83-89).
+
+        Args:
+            filename (str): Output file name.
+            use_longid (bool): Write 8-byte ID (default: False)
+        """
+        c._write_gadget_binary(self._particles, filename, use_longid)
+
+    def save_hdf5(self, filename, var):
+        """HDF5 snapshot (py/fs/particles.py:91-104).
+
+        Args:
+            filename (str): Output file name.
+            var (str): Output variables, a subset of 'ixvcf12' in this order.
+
+                * i: ID
+                * x: positions
+                * v: velocities (c: COLA velocities, written as v)
+                * f: force
+                * 1: 1LPT displacements (a=1)
+                * 2: 2nd-order displacements (a=1)
+        """
+        c._hdf5_write_particles(self._particles, filename, var)
+
     def copy_from(self, other):
         """Device-to-device copy of all state (same capacity)."""
         c._particles_copy(self._particles, other._particles)

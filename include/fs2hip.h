@@ -99,6 +99,8 @@ int fs2_cosmology_f_growth_rate(double a, double* out);
 int fs2_cosmology_Dv_growth(double a, double* out);
 /* py/cosmology.cpp:104-108 cosmology_hubble_function */
 int fs2_cosmology_hubble(double a, double* out);
+/* Omega_m of fs2_cosmology_init (py/cosmology.cpp cosmology_omega_m) */
+int fs2_cosmology_omega_m(double* out);
 /* py/cola.cpp:57-66: FP64 kick factor, q1 = D1(a_x), q2 = D2 - D1^2. */
 int fs2_cola_kick_coefficients(double a_v, double a_x, double a_vel,
                                double* kick_factor, double* q1, double* q2);
@@ -225,6 +227,14 @@ int fs2_cola_kick(fs2_particles* p, double a_vel);
 int fs2_cola_drift(fs2_particles* p, double a_pos);
 /* py/cola.cpp:129-148 cola_velocity: (np_local, 3) FP32 into out */
 int fs2_cola_velocity(fs2_particles* p, float* out);
+
+/* ---- output ------------------------------------------------------------------ */
+/* py/gadget_file.cpp:15-114 / py_write.cpp _write_gadget_binary: Gadget-1
+ * binary snapshot (256-byte header, positions, velocities / sqrt(a),
+ * 4- or 8-byte ids) of one rank's particles.  h = hubble_param of the
+ * header (1.0 in the reference's call).  FS2_ERR_IO if the file cannot be
+ * written.  The HDF5 writer lives in libfs2io.so (include/fs2io.h). */
+int fs2_write_gadget(fs2_particles* p, const char* filename, int use_long_id, double h);
 
 /* ---- standard leapfrog ----------------------------------------------------- */
 /* py/leapfrog.cpp:96-141 SphiStd(ai, af) and SqStd(ai, af): the kick factor

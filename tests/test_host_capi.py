@@ -21,8 +21,8 @@ LIB = os.path.join(ROOT, "fs2_amd", "libfs2hip.so")
 TABLE = os.path.join(ROOT, "tests", "golden", "planck_matterpower.dat")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(fs2_[a-z0-9_]+)\s*\(", text)))
 
@@ -33,6 +33,18 @@ def test_library_exports_every_declared_symbol():
     assert len(names) > 50
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_io_library_exports_every_declared_symbol():
+    """libfs2io.so (HDF5 output, built where an HDF5 C library exists)."""
+    iolib = os.path.join(ROOT, "fs2_amd", "libfs2io.so")
+    if not os.path.exists(iolib):
+        pytest.skip("libfs2io.so not built (no HDF5 C library)")
+    import torch  # noqa: F401  (one HIP runtime: the one libfs2hip binds in the package)
+    lib = ctypes.CDLL(iolib)
+    names = declared_functions(os.path.join(ROOT, "include", "fs2io.h"))
+    assert names == ["fs2_write_hdf5"]
+    assert hasattr(lib, "fs2_write_hdf5")
 
 
 def test_extension_method_table_matches_reference_names(fs):
@@ -49,6 +61,7 @@ def test_extension_method_table_matches_reference_names(fs):
         "_pm_check_total_density", "_pm_get_forces", "_pm_set_packet_size", "_cola_kick",
         "_cola_drift", "_fft_alloc", "_fft_set_test_data", "_fft_fx_global_as_array",
         "_leapfrog_initial_velocity", "_leapfrog_kick", "_leapfrog_drift",
+        "_write_gadget_binary", "_hdf5_write_particles",
         "config_precision"]
     missing = [n for n in expected if not hasattr(fs._fs, n)]
     assert not missing, missing
