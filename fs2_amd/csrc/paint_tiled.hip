@@ -427,12 +427,31 @@ __device__ __forceinline__ void stage_halo(float* hs, int* rowoff, const float* 
     rowoff[r] = (gx * g.n + gy) * g.ncz;   // < 2^31 for meshes up to 1024^3
   }
   __syncthreads();
+  // U elements per thread per round, all loads issued before any LDS store,
+  // so a 1024-thread block keeps ~U * NM loads per thread in flight
+  constexpr int U = 6;
   const int interior = rows << g.lz;
-  for (int i = threadIdx.x; i < interior; i += blockDim.x) {
-    const int r = i >> g.lz, lz = i & (g.tz - 1);
-    const int src = rowoff[r] + z0 + lz;
+  for (int i0 = threadIdx.x; i0 < interior; i0 += U * blockDim.x) {
+    float v[U][NM];
+    int dst[U];
 #pragma unroll
-    for (int a = 0; a < NM; ++a) hs[a * hcells + r * hz + lz] = ms[a][src];
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * blockDim.x;
+      dst[u] = -1;
+      if (i < interior) {
+        const int r = i >> g.lz, lz = i & (g.tz - 1);
+        const int src = rowoff[r] + z0 + lz;
+#pragma unroll
+        for (int a = 0; a < NM; ++a) v[u][a] = ms[a][src];
+        dst[u] = r * hz + lz;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (dst[u] >= 0) {
+#pragma unroll
+        for (int a = 0; a < NM; ++a) hs[a * hcells + dst[u]] = v[u][a];
+      }
   }
   int gz = z0 + g.tz;
   if (gz >= g.n) gz -= g.n;
