@@ -108,10 +108,10 @@ inline size_t grid_floats(int n) { return (size_t)n * n * grid_rpitch(n); }
 // Workgroups are dispatched round-robin over the 8 XCDs of the MI355X; this
 // remaps the hardware block index so that consecutive logical blocks run on
 // the same XCD and share its L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
-  const int per = nblocks / 8, rem = nblocks % 8;
-  const int xcd = bid % 8, local = bid / 8;
-  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+__device__ __forceinline__ int xcd_remap(unsigned bid, unsigned nblocks) {
+  const unsigned per = nblocks >> 3, rem = nblocks & 7u;
+  const unsigned xcd = bid & 7u, local = bid >> 3;
+  return (int)(xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local);
 }
 
 // Green x gradient multiplier f2 = f1 k_a / |k|^2 (py/pm.cpp:460-504) with a

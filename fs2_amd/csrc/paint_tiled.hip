@@ -26,6 +26,7 @@
 // disabled, exactly as the per-particle atomic kernel does.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "fs2_internal.h"
 
@@ -82,7 +83,37 @@ struct TileGeom {
   int ntiles, nbox;
   unsigned npb, bins_box;
   int64_t mstride;
+  // log2 of ntz, nty and ntiles when all three are powers of two, else -1
+  int lntz, lnty, lnt;
 };
+
+// Tile of workgroup b (box b / ntiles, tile b % ntiles, z fastest); shifts
+// when the tile counts are powers of two, unsigned divisions otherwise.
+struct TileCoord {
+  int box, tile, Tx, Ty, Tz;
+};
+__device__ __forceinline__ TileCoord tile_coord(const TileGeom& g, unsigned b) {
+  TileCoord c;
+  unsigned box, tile, q;
+  if (g.lnt >= 0) {
+    box = b >> g.lnt;
+    tile = b & ((1u << g.lnt) - 1u);
+    c.Tz = (int)(tile & ((1u << g.lntz) - 1u));
+    q = tile >> g.lntz;
+    c.Ty = (int)(q & ((1u << g.lnty) - 1u));
+    c.Tx = (int)(q >> g.lnty);
+  } else {
+    box = b / (unsigned)g.ntiles;
+    tile = b - box * (unsigned)g.ntiles;
+    q = tile / (unsigned)g.ntz;
+    c.Tz = (int)(tile - q * (unsigned)g.ntz);
+    c.Tx = (int)(q / (unsigned)g.nty);
+    c.Ty = (int)(q - (unsigned)c.Tx * (unsigned)g.nty);
+  }
+  c.box = (int)box;
+  c.tile = (int)tile;
+  return c;
+}
 
 __device__ __forceinline__ unsigned key_of(const TileGeom& g, int ix, int iy, int iz) {
   const unsigned tile = ((unsigned)(ix >> g.lx) * g.nty + (unsigned)(iy >> g.ly)) * g.ntz +
@@ -95,23 +126,31 @@ __device__ __forceinline__ unsigned key_of(const TileGeom& g, int ix, int iy, in
 
 // Lanes of a wavefront grouped by equal key: on return *leader is the
 // lowest lane holding this lane's key and *same the mask of those lanes.
-// One ballot round per distinct key; no memory traffic.
+// One round per distinct key: the key of the lowest ungrouped lane is read
+// into a scalar register and compared by every lane; no memory traffic.
 __device__ __forceinline__ void group_by_key(unsigned key, bool valid, int* leader,
                                              uint64_t* same) {
   uint64_t active = __ballot(valid);
-  const int lane = __lane_id();
-  *leader = lane;
-  *same = 0;
+  int ld = __lane_id();
+  uint64_t sm = 0;
   while (active) {
-    const int l = __ffsll((unsigned long long)active) - 1;
-    const unsigned lk = __shfl(key, l);
-    const uint64_t m = __ballot(valid && key == lk) & active;
-    if ((m >> lane) & 1ull) {
-      *leader = l;
-      *same = m;
+    const int l = __builtin_ctzll(active);
+    const unsigned lk = (unsigned)__builtin_amdgcn_readlane((int)key, l);
+    const bool mine = valid && key == lk;
+    const uint64_t m = __ballot(mine);
+    if (mine) {
+      ld = l;
+      sm = m;
     }
     active &= ~m;
   }
+  *leader = ld;
+  *same = sm;
+}
+
+// Number of lanes below this one in mask m.
+__device__ __forceinline__ unsigned lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
 // Wavefront-aggregated atomicAdd of 1 per lane to ctr[key]; returns the
@@ -126,7 +165,7 @@ __device__ __forceinline__ unsigned wave_aggregated_inc(unsigned* ctr, unsigned 
   unsigned base = 0;
   if (valid && lane == leader) base = atomicAdd(ctr + key, (unsigned)__popcll(same));
   base = __shfl(base, leader);
-  return base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+  return base + lanes_below(same);
 }
 
 // Counting only: one atomic per distinct key and wavefront, result unused.
@@ -276,35 +315,34 @@ __global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t n
 
 // Source tiles S = T - d, d in {0,1}^3 (periodic).  A particle of S
 // reaches T only by spilling along every axis where S != T, so S
-// contributes the classes c with (c & need) == need.  Wavefront 0 examines
-// the 64 (d, class) pairs in parallel, one per lane, and compacts the
-// non-empty bin ranges: range r covers records [rb[r], rb[r] + rpre[r+1] -
-// rpre[r]), rpre is their exclusive prefix sum and rpre[nranges] the total.
-// rb needs 64 entries, rpre 65.  Block-cooperative; ends with a barrier.
+// contributes the classes c with (c & d) == d.  An offset along an axis with
+// a single tile names T's own column again; such d are skipped, their
+// particles being those of the d without that bit.  The bins of one tile
+// are contiguous, so a run of consecutive selected classes is one record
+// range: at most 14 ranges, the first (d = 0) holding all of T's own
+// records.  Wavefront 0 examines the 64 (d, class) pairs, one per lane, and
+// compacts the run starts: range r covers records [rb[r], rb[r] + rpre[r+1]
+// - rpre[r]), rpre is their exclusive prefix sum and rpre[nranges] the
+// total.  rb needs 64 entries, rpre 65.  Block-cooperative; ends with a
+// barrier.
 __device__ __forceinline__ void tile_source_ranges(const unsigned* __restrict__ offsets,
                                                    const TileGeom& g, int Tx, int Ty, int Tz,
                                                    unsigned* rb, unsigned* rpre, int* nranges) {
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
     const int d = lane >> 3, cl = lane & 7;
-    auto src = [&](int dd, int* Sx, int* Sy, int* Sz) {
-      *Sx = (Tx - ((dd >> 2) & 1) + g.ntx) % g.ntx;
-      *Sy = (Ty - ((dd >> 1) & 1) + g.nty) % g.nty;
-      *Sz = (Tz - (dd & 1) + g.ntz) % g.ntz;
-      return (*Sx * g.nty + *Sy) * g.ntz + *Sz;
-    };
-    int Sx, Sy, Sz;
-    const int S = src(d, &Sx, &Sy, &Sz);
-    bool dup = false;   // a periodic wrap can make two offsets name one tile
-    for (int e = 0; e < d; ++e) {
-      int ex, ey, ez;
-      dup |= src(e, &ex, &ey, &ez) == S;
-    }
-    const unsigned need = (Sx != Tx ? 4u : 0u) | (Sy != Ty ? 2u : 0u) | (Sz != Tz ? 1u : 0u);
+    const int single = (g.ntx == 1 ? 4 : 0) | (g.nty == 1 ? 2 : 0) | (g.ntz == 1 ? 1 : 0);
+    auto sel = [d](int c) { return (c & d) == d; };
     unsigned b = 0, len = 0;
-    if (!dup && ((unsigned)cl & need) == need) {
-      b = offsets[(unsigned)S * 8u + cl];
-      const unsigned e = offsets[(unsigned)S * 8u + cl + 1];
+    if ((d & single) == 0 && sel(cl) && (cl == 0 || !sel(cl - 1))) {
+      int end = cl;
+      while (end < 7 && sel(end + 1)) ++end;
+      const unsigned Sx = (d & 4) ? (Tx ? Tx - 1 : g.ntx - 1) : Tx;
+      const unsigned Sy = (d & 2) ? (Ty ? Ty - 1 : g.nty - 1) : Ty;
+      const unsigned Sz = (d & 1) ? (Tz ? Tz - 1 : g.ntz - 1) : Tz;
+      const unsigned S8 = ((Sx * (unsigned)g.nty + Sy) * (unsigned)g.ntz + Sz) * 8u;
+      b = offsets[S8 + cl];
+      const unsigned e = offsets[S8 + end + 1];
       len = e > b ? e - b : 0u;
     }
     unsigned incl = len;
@@ -327,6 +365,55 @@ __device__ __forceinline__ void tile_source_ranges(const unsigned* __restrict__ 
   __syncthreads();
 }
 
+// Zeroes the first `bytes` bytes of LDS (a multiple of 4) with 16-byte
+// stores and a word-wise tail.  The caller's next barrier orders it.
+__device__ __forceinline__ void lds_zero(unsigned char* smem, int bytes) {
+  uint4* a4 = (uint4*)smem;
+  const int n16 = bytes >> 4;
+  for (int i = threadIdx.x; i < n16; i += blockDim.x) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+  const int tail = (bytes & 15) >> 2;
+  if ((int)threadIdx.x < tail) ((unsigned*)(smem + (n16 << 4)))[threadIdx.x] = 0u;
+}
+
+// Offset of owned cell c (z fastest within the tile) from the tile's origin
+// cell in a padded mesh: (lx * n + ly) * ncz + lz, below 2^31 for any tile.
+__device__ __forceinline__ int tile_cell_offset(const TileGeom& g, int c) {
+  const int lz = c & (g.tz - 1);
+  const int row = c >> g.lz;
+  return ((row >> g.ly) * g.n + (row & (g.ty - 1))) * g.ncz + lz;
+}
+
+// Visits the owned cells of a tile in groups of V = 4, 2 or 1 consecutive z
+// cells, fn(c, offset, V) with c the first cell.  Rows of the padded mesh
+// start at multiples of 32 floats and z0 is a multiple of tz, so a group
+// of V cells is one aligned V-float store when V divides tz.
+template <typename Fn>
+__device__ __forceinline__ void for_tile_groups(const TileGeom& g, Fn fn) {
+  const int tcells = g.tx * g.ty * g.tz;
+  if ((g.tz & 3) == 0) {
+    for (int c = 4 * threadIdx.x; c < tcells; c += 4 * blockDim.x)
+      fn(c, tile_cell_offset(g, c), std::integral_constant<int, 4>());
+  } else if ((g.tz & 1) == 0) {
+    for (int c = 2 * threadIdx.x; c < tcells; c += 2 * blockDim.x)
+      fn(c, tile_cell_offset(g, c), std::integral_constant<int, 2>());
+  } else {
+    for (int c = threadIdx.x; c < tcells; c += blockDim.x)
+      fn(c, tile_cell_offset(g, c), std::integral_constant<int, 1>());
+  }
+}
+
+// Stores V consecutive floats v[0..V) at p (V-float aligned).
+template <int V>
+__device__ __forceinline__ void store_group(float* p, const float* v) {
+  if constexpr (V == 4) {
+    *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+  } else if constexpr (V == 2) {
+    *(float2*)p = make_float2(v[0], v[1]);
+  } else {
+    *p = v[0];
+  }
+}
+
 // K4: one workgroup per tile.  ACC is float (LDS float atomics) or
 // unsigned long long (int64 fixed point, deterministic).
 template <typename ACC>
@@ -336,26 +423,22 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
   extern __shared__ unsigned char smem[];
   ACC* acc = (ACC*)smem;
   const int tcells = g.tx * g.ty * g.tz;
-  for (int c = threadIdx.x; c < tcells; c += blockDim.x) acc[c] = (ACC)0;
-  __syncthreads();
+  lds_zero(smem, tcells * (int)sizeof(ACC));
 
   // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int box = bid / g.ntiles;
-  const int tile = bid - box * g.ntiles;
-  offsets += (int64_t)box * g.bins_box;
-  const int64_t moff = (int64_t)box * g.mstride;
-  const int Tz = tile % g.ntz;
-  const int Ty = (tile / g.ntz) % g.nty;
-  const int Tx = tile / (g.ntz * g.nty);
-  const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
-  grid += moff;
+  const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
+  offsets += (int64_t)tc.box * g.bins_box;
+  const int x0 = tc.Tx * g.tx, y0 = tc.Ty * g.ty, z0 = tc.Tz * g.tz;
+  float* gt = grid + (int64_t)tc.box * g.mstride + ((int64_t)x0 * g.n + y0) * g.ncz + z0;
 
   __shared__ unsigned rb[64], rpre[65];
   __shared__ int nranges;
-  tile_source_ranges(offsets, g, Tx, Ty, Tz, rb, rpre, &nranges);
+  tile_source_ranges(offsets, g, tc.Tx, tc.Ty, tc.Tz, rb, rpre, &nranges);
   const int nr = nranges;
   const unsigned total = rpre[nr];
+  // fixed point: fix is a power of two, so (fac fix) wx wy wz rounds exactly
+  // as (fac wx wy wz) fix
+  const float f0 = sizeof(ACC) == 4 ? fac : fac * fix;
   int r = 0;
   for (unsigned v = threadIdx.x; v < total; v += blockDim.x) {
     while (v >= rpre[r + 1]) ++r;
@@ -380,31 +463,32 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
         for (int ez = 0; ez < 2; ++ez) {
           if ((unsigned)lz[ez] >= (unsigned)g.tz) continue;
           // fac*wx*wy*wz, left to right (pm.cpp:122-132)
-          const float w = fac * wx[ex] * wy[ey] * wz[ez];
+          const float w = f0 * wx[ex] * wy[ey] * wz[ez];
           const int li = (((lx[ex] << g.ly) + ly[ey]) << g.lz) + lz[ez];
           if constexpr (sizeof(ACC) == 4) {
             atomicAdd(&acc[li], w);
           } else {
-            // w <= fac, fix * fac <= 2^30: one rounding to int32 per term
-            atomicAdd(&acc[li], (ACC)(long long)__float2int_rn(w * fix));
+            // w <= fac fix <= 2^30: one rounding to int32 per term
+            atomicAdd(&acc[li], (ACC)(long long)__float2int_rn(w));
           }
         }
       }
     }
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < tcells; c += blockDim.x) {
-    const int lz = c & (g.tz - 1);
-    const int ly = (c >> g.lz) & (g.ty - 1);
-    const int lx = c >> (g.lz + g.ly);
-    float v;
-    if constexpr (sizeof(ACC) == 4) {
-      v = acc[c];
-    } else {
-      v = (float)((double)(long long)acc[c] * fix_inv);
+  for_tile_groups(g, [&](int c, int off, auto vc) {
+    constexpr int V = decltype(vc)::value;
+    float o[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      if constexpr (sizeof(ACC) == 4) {
+        o[k] = acc[c + k] - 1.0f;
+      } else {
+        o[k] = (float)((double)(long long)acc[c + k] * fix_inv) - 1.0f;
+      }
     }
-    grid[((int64_t)(x0 + lx) * g.n + (y0 + ly)) * g.ncz + (z0 + lz)] = v - 1.0f;
-  }
+    store_group<V>(gt + off, o);
+  });
 }
 
 // Copies the (tx+1) x (ty+1) x (tz+1) halo block of NM meshes starting at
@@ -483,14 +567,11 @@ k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offs
   const int hx = g.tx + 1, hy = g.ty + 1, hz = g.tz + 1;
   const int hcells = hx * hy * hz;
   // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int box = bid / g.ntiles;
-  const int tile = bid - box * g.ntiles;
-  offsets += (int64_t)box * g.bins_box;
-  const int64_t moff = (int64_t)box * g.mstride;
-  const int Tz = tile % g.ntz;
-  const int Ty = (tile / g.ntz) % g.nty;
-  const int Tx = tile / (g.ntz * g.nty);
+  const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
+  const unsigned tile = (unsigned)tc.tile;
+  offsets += (int64_t)tc.box * g.bins_box;
+  const int64_t moff = (int64_t)tc.box * g.mstride;
+  const int Tx = tc.Tx, Ty = tc.Ty, Tz = tc.Tz;
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
   const float* ms[3] = {m0 + moff, m1 + moff, m2 + moff};
   stage_halo<3>(hs, (int*)(hs + 3 * hcells), ms, g, x0, y0, z0);
@@ -570,17 +651,14 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
   extern __shared__ unsigned char smem[];
   unsigned long long* acc = (unsigned long long*)smem;
   const int tcells = g.tx * g.ty * g.tz;
-  for (int c = threadIdx.x; c < 3 * tcells; c += blockDim.x) acc[c] = 0ull;
+  lds_zero(smem, 3 * tcells * (int)sizeof(unsigned long long));
 
   // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int box = bid / g.ntiles;
-  const int tile = bid - box * g.ntiles;
-  offsets += (int64_t)box * g.bins_box;
-  const int64_t moff = (int64_t)box * g.mstride;
-  const int Tz = tile % g.ntz;
-  const int Ty = (tile / g.ntz) % g.nty;
-  const int Tx = tile / (g.ntz * g.nty);
+  const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
+  const unsigned tile = (unsigned)tc.tile;
+  offsets += (int64_t)tc.box * g.bins_box;
+  const int64_t moff = (int64_t)tc.box * g.mstride;
+  const int Tx = tc.Tx, Ty = tc.Ty, Tz = tc.Tz;
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
   float sc[3];
 #pragma unroll
@@ -630,16 +708,22 @@ k_tile_paint_values3(const float4* __restrict__ rec, const unsigned* __restrict_
     }
   }
   __syncthreads();
-  float* outs[3] = {o0 + moff, o1 + moff, o2 + moff};
-  for (int c = threadIdx.x; c < tcells; c += blockDim.x) {
-    const int lz = c & (g.tz - 1);
-    const int ly = (c >> g.lz) & (g.ty - 1);
-    const int lx = c >> (g.lz + g.ly);
-    const int64_t gi = ((int64_t)(x0 + lx) * g.n + (y0 + ly)) * g.ncz + (z0 + lz);
+  const int64_t t0 = moff + ((int64_t)x0 * g.n + y0) * g.ncz + z0;
+  float* outs[3] = {o0 + t0, o1 + t0, o2 + t0};
+  // sc is a power of two: multiplying by 1/sc is the exact division
+  double isc[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-      outs[a][gi] = (float)((double)(long long)acc[a * tcells + c] / (double)sc[a]);
-  }
+  for (int a = 0; a < 3; ++a) isc[a] = 1.0 / (double)sc[a];
+  for_tile_groups(g, [&](int c, int off, auto vc) {
+    constexpr int V = decltype(vc)::value;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float o[V];
+#pragma unroll
+      for (int k = 0; k < V; ++k) o[k] = (float)((double)(long long)acc[a * tcells + c + k] * isc[a]);
+      store_group<V>(outs[a] + off, o);
+    }
+  });
 }
 
 // Gradient readout at the tile's own particles:
@@ -659,14 +743,11 @@ k_tile_grad_readout(const float4* __restrict__ rec, const unsigned* __restrict__
   const int hx = g.tx + 1, hy = g.ty + 1, hz = g.tz + 1;
   const int hcells = hx * hy * hz;
   // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int box = bid / g.ntiles;
-  const int tile = bid - box * g.ntiles;
-  offsets += (int64_t)box * g.bins_box;
-  const int64_t moff = (int64_t)box * g.mstride;
-  const int Tz = tile % g.ntz;
-  const int Ty = (tile / g.ntz) % g.nty;
-  const int Tx = tile / (g.ntz * g.nty);
+  const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
+  const unsigned tile = (unsigned)tc.tile;
+  offsets += (int64_t)tc.box * g.bins_box;
+  const int64_t moff = (int64_t)tc.box * g.mstride;
+  const int Tx = tc.Tx, Ty = tc.Ty, Tz = tc.Tz;
   const int x0 = Tx * g.tx, y0 = Ty * g.ty, z0 = Tz * g.tz;
   const float* ms[4] = {rhobar + moff, FORCES ? m0 + moff : nullptr, FORCES ? m1 + moff : nullptr,
                         FORCES ? m2 + moff : nullptr};
@@ -733,6 +814,14 @@ TileGeom make_geom(int n, float boxsize, const TileOpts& o, int64_t np = 0) {
   g.ntz = n / g.tz;
   g.dx_inv = (float)n / boxsize;   // pm.cpp:77
   g.ntiles = g.ntx * g.nty * g.ntz;
+  auto pow2 = [](int v) { return (v & (v - 1)) == 0; };
+  if (pow2(g.ntx) && pow2(g.nty) && pow2(g.ntz)) {
+    g.lntz = __builtin_ctz((unsigned)g.ntz);
+    g.lnty = __builtin_ctz((unsigned)g.nty);
+    g.lnt = __builtin_ctz((unsigned)g.ntiles);
+  } else {
+    g.lntz = g.lnty = g.lnt = -1;
+  }
   g.bins_box = (unsigned)g.ntiles * 8u;
   g.nbox = o.nbox < 1 ? 1 : o.nbox;
   if (np % g.nbox) fail(FS2_ERR_VALUE, "%lld particles do not split into %d boxes",
