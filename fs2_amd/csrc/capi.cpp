@@ -27,6 +27,15 @@ struct fs2_particles {
   uint64_t* id = nullptr;
   bool wrap_pending = false;
   uint64_t x_version = 0;    // changes whenever positions change (see touch_x)
+  // A COLA kick waiting to run with the next drift as one fused pass; any
+  // other access to the particles applies it first (apply_kick).  Once the
+  // device planes have been handed out, kicks run at once.
+  struct {
+    bool pending = false;
+    double m15om = 0.0;
+    float kf = 0.f, q1 = 0.f, q2 = 0.f;
+  } kick;
+  bool planes_exposed = false;
   float* field(int k) const { return planes + (size_t)k * 3 * np_alloc; }
   float* x() const { return field(0); }
   float* v() const { return field(1); }
@@ -228,8 +237,18 @@ void set_profiling(bool on) { g_prof = on; }
 namespace {
 
 
+void apply_kick(fs2_particles* p) {
+  if (!p->kick.pending) return;
+  p->kick.pending = false;
+  cola_kick(p->v(), p->f(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local,
+            p->kick.m15om, p->kick.kf, p->kick.q1, p->kick.q2);
+}
+
+// Validates the handle and brings the device planes up to date: every entry
+// point that reads or writes particle data goes through here.
 fs2_particles* check_p(fs2_particles* p) {
   if (!p) fail(FS2_ERR_VALUE, "null particles handle");
+  apply_kick(p);
   return p;
 }
 
@@ -648,6 +667,7 @@ int fs2_particles_device_fields(fs2_particles* p, float** x, float** v, float** 
     if (id) *id = p->id;
     if (stride) *stride = p->np_alloc;
     touch_x(p);   // the caller may write the planes
+    p->planes_exposed = true;
   });
 }
 
@@ -666,6 +686,7 @@ int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src) {
     dst->a_v = src->a_v;
     dst->a_f = src->a_f;
     dst->wrap_pending = src->wrap_pending;
+    dst->kick = src->kick;   // the copied planes still owe src's pending kick
     touch_x(dst);
   });
 }
@@ -878,30 +899,43 @@ int fs2_pm_power_spectrum(fs2_pm* pm, int nbins, double dk_bin, double* k, doubl
 }
 
 // ---- COLA ---------------------------------------------------------------------------------------------------------
+// The kick is recorded and, when a drift follows, fused with it into one
+// pass over the particles (cola_kick_drift: the same two updates in order,
+// so the result is bitwise that of kick then drift).
 int fs2_cola_kick(fs2_particles* p, double a_vel) {
   return guard([&] {
-    check_p(p);
+    check_p(p);   // applies an earlier pending kick
     double kf, q1, q2;
     int rc = fs2_cola_kick_coefficients(p->a_v, p->a_x, a_vel, &kf, &q1, &q2);
     if (rc) throw Error{rc, g_err};
-    const double om = host::omega_m();
     // cola.cpp:57, 65-66: kick_factor, q1, q2 stored as Float
-    cola_kick(p->v(), p->f(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local,
-              -1.5 * om, (float)kf, (float)q1, (float)q2);
+    p->kick.m15om = -1.5 * host::omega_m();
+    p->kick.kf = (float)kf;
+    p->kick.q1 = (float)q1;
+    p->kick.q2 = (float)q2;
+    p->kick.pending = true;
+    if (p->planes_exposed) apply_kick(p);
     p->a_v = a_vel;
   });
 }
 
 int fs2_cola_drift(fs2_particles* p, double a_pos) {
   return guard([&] {
-    check_p(p);
+    if (!p) fail(FS2_ERR_VALUE, "null particles handle");
     double dt, da1, da2;
     int rc = fs2_cola_drift_coefficients(p->a_x, p->a_v, a_pos, &dt, &da1, &da2);
     if (rc) throw Error{rc, g_err};
     flush_wrap(p);
     // cola.cpp:106-109: da1, da2 stored as Float; dt stays double
-    cola_drift(p->x(), p->v(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local, dt,
-               (float)da1, (float)da2);
+    if (p->kick.pending) {
+      p->kick.pending = false;
+      cola_kick_drift(p->x(), p->v(), p->f(), p->dx1(), p->dx2(), (int64_t)p->np_alloc,
+                      (int64_t)p->np_local, p->kick.m15om, p->kick.kf, p->kick.q1, p->kick.q2,
+                      dt, (float)da1, (float)da2);
+    } else {
+      cola_drift(p->x(), p->v(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local,
+                 dt, (float)da1, (float)da2);
+    }
     touch_x(p);
     p->a_x = a_pos;
   });

@@ -46,6 +46,31 @@ __global__ void k_drift(float* __restrict__ x, const float* __restrict__ v,
   }
 }
 
+// k_kick followed by k_drift on the same particle, fused: v is written by
+// the kick expression and that rounded FP32 value feeds the drift, exactly
+// as when the two kernels run in sequence.  One pass reads x v f dx1 dx2
+// and writes x v.
+__global__ void k_kick_drift(float* __restrict__ x, float* __restrict__ v,
+                             const float* __restrict__ f, const float* __restrict__ dx1,
+                             const float* __restrict__ dx2, int64_t stride, int64_t np,
+                             double m15om, float kf, float q1, float q2, double dt, float da1,
+                             float da2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t j = i + a * stride;
+    const float d1 = dx1[j], d2 = dx2[j];
+    const float s = f[j] + d1 * q1 + d2 * q2;
+    const float acc = (float)(m15om * (double)s);
+    const float vn = v[j] + acc * kf;
+    v[j] = vn;
+    const float lpt = d1 * da1 + d2 * da2;
+    const double rhs = (double)vn * dt + (double)lpt;
+    x[j] = (float)((double)x[j] + rhs);
+  }
+}
+
 // cola.cpp:141-145: v + Dv dx1 + D2v dx2 (Float coefficients).
 __global__ void k_velocity(float* __restrict__ out, const float* __restrict__ v,
                            const float* __restrict__ dx1, const float* __restrict__ dx2,
@@ -138,6 +163,14 @@ void cola_drift(float* x, const float* v, const float* dx1, const float* dx2, in
   if (np == 0) return;
   launch("cola_drift", (double)np * 60.0, k_drift, dim3(blocks_for(np)), dim3(kBlock), 0, x, v,
          dx1, dx2, stride, np, dt, da1, da2);
+}
+
+void cola_kick_drift(float* x, float* v, const float* f, const float* dx1, const float* dx2,
+                     int64_t stride, int64_t np, double m15om, float kf, float q1, float q2,
+                     double dt, float da1, float da2) {
+  if (np == 0) return;
+  launch("cola_kick_drift", (double)np * 84.0, k_kick_drift, dim3(blocks_for(np)), dim3(kBlock),
+         0, x, v, f, dx1, dx2, stride, np, m15om, kf, q1, q2, dt, da1, da2);
 }
 
 void cola_velocity(float* out, const float* v, const float* dx1, const float* dx2,

@@ -206,6 +206,10 @@ void cola_kick(float* v, const float* f, const float* dx1, const float* dx2,
                int64_t stride, int64_t np, double m15om, float kf, float q1, float q2);
 void cola_drift(float* x, const float* v, const float* dx1, const float* dx2,
                 int64_t stride, int64_t np, double dt, float da1, float da2);
+// kick then drift in one pass (bitwise equal to the two calls in sequence)
+void cola_kick_drift(float* x, float* v, const float* f, const float* dx1, const float* dx2,
+                     int64_t stride, int64_t np, double m15om, float kf, float q1, float q2,
+                     double dt, float da1, float da2);
 void cola_velocity(float* out, const float* v, const float* dx1, const float* dx2,
                    int64_t stride, int64_t np, float Dv, float D2v);
 void scale_field(float* out, const float* in, int64_t stride, int64_t np, float s);

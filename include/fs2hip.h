@@ -164,7 +164,8 @@ int fs2_particles_get_times(const fs2_particles* p, double* a_x, double* a_v,
                             double* a_f);
 int fs2_particles_set_times(fs2_particles* p, double a_x, double a_v, double a_f);
 /* Device planes [3][stride] of each field and the id array (for callers
- * that launch their own work, e.g. autograd).  Synchronises pending wraps. */
+ * that launch their own work, e.g. autograd).  Applies pending wraps and
+ * kicks; from then on fs2_cola_kick on these particles runs at once. */
 int fs2_particles_device_fields(fs2_particles* p, float** x, float** v,
                                 float** dx1, float** dx2, float** force,
                                 uint64_t** id, uint64_t* stride);
@@ -221,9 +222,12 @@ int fs2_pm_power_spectrum(fs2_pm* pm, int nbins, double dk_bin, double* k,
                           double* P, double* nmodes);
 
 /* ---- COLA ---------------------------------------------------------------- */
-/* py/cola.cpp:48-92 cola_kick */
+/* py/cola.cpp:48-92 cola_kick.  The coefficients are computed (and checked)
+ * here; the velocity update is deferred and fused with the next
+ * fs2_cola_drift into one pass (bitwise the same result).  Any other call
+ * that reads or writes these particles applies a deferred kick first. */
 int fs2_cola_kick(fs2_particles* p, double a_vel);
-/* py/cola.cpp:94-127 cola_drift */
+/* py/cola.cpp:94-127 cola_drift (runs a deferred kick in the same pass) */
 int fs2_cola_drift(fs2_particles* p, double a_pos);
 /* py/cola.cpp:129-148 cola_velocity: (np_local, 3) FP32 into out */
 int fs2_cola_velocity(fs2_particles* p, float* out);

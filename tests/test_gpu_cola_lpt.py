@@ -100,6 +100,37 @@ def test_kick_drift_match_oracle(fs, cosmo):
     assert p.times[0] == 0.3
 
 
+def test_deferred_kick_is_bitwise_kick_then_drift(fs, cosmo, eh):
+    """fs.cola.kick defers its velocity update into the next drift (one fused
+    pass).  Whatever touches the particles in between (reading v, another
+    pm.force, a copy) applies the kick first; every order ends bitwise equal."""
+    _announce("deferred kick: fused vs separate passes")
+    n_p, nc, L = 16, 32, 64.0
+    fs.pm.init(nc, nc / n_p, L)
+    fs.pm.set_paint_variant("sorted")
+    out = {}
+    for mode in ("fused", "read_v", "force_between", "copy"):
+        p = fs.lpt.init(n_p, L, 0.1, eh, 5, "cola")
+        for a_vel, a_pos in ocola.schedule(0.1, 1.0, 4):
+            fs.pm.force(p)
+            fs.cola.kick(p, a_vel)
+            if mode == "read_v":
+                _ = p.v
+            elif mode == "force_between":
+                fs.pm.force(p)      # same positions: same forces
+            elif mode == "copy":
+                q = fs.Particles(len(p), L)
+                q.copy_from(p)      # q owes p's pending kick
+                fs.cola.drift(q, a_pos)
+                p.copy_from(q)
+                continue
+            fs.cola.drift(p, a_pos)
+        out[mode] = (p.x, p.v)
+    for mode in ("read_v", "force_between", "copy"):
+        for k in range(2):
+            assert np.array_equal(out[mode][k], out["fused"][k]), (mode, k)
+
+
 def test_leapfrog_kernels_match_oracle(fs, cosmo):
     """py/leapfrog.cpp:18-94: initial velocity, kick, drift."""
     from oracle import leapfrog as olf
