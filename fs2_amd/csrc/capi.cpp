@@ -941,6 +941,115 @@ int fs2_cola_drift(fs2_particles* p, double a_pos) {
   });
 }
 
+// ---- friends-of-friends halo finder (py/fof.cpp, py/kdtree.cpp) ---------------------------
+namespace {
+// Result of the last fs2_fof_find_groups (the reference keeps the group
+// array and the tree in file-scope state as well, fof.cpp:10-17).
+FofResult g_fof;
+bool g_fof_done = false;
+
+struct PinnedHost {
+  void* p = nullptr;
+  explicit PinnedHost(size_t bytes) {
+    device_init();
+    FS2_HIP(hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault));
+  }
+  ~PinnedHost() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+struct DeviceTmp {
+  void* p;
+  explicit DeviceTmp(size_t bytes) : p(dmalloc(bytes)) {}
+  ~DeviceTmp() { dfree(p); }
+};
+}  // namespace
+
+int fs2_fof_find_groups(fs2_particles* p, double linking_length, const double* boxsize3,
+                        int quota, uint64_t* ngroups) {
+  return guard([&] {
+    check_p(p);
+    flush_wrap(p);
+    if (!(linking_length >= 0.0)) fail(FS2_ERR_VALUE, "linking length must be >= 0");
+    const int64_t n = (int64_t)p->np_local;
+    // final positions to the host through pinned memory
+    PinnedHost h((size_t)n * 3 * sizeof(float) + (size_t)n * sizeof(int32_t));
+    float* hx = (float*)h.p;
+    for (int a = 0; a < 3; ++a)
+      if (n > 0)
+        FS2_HIP(hipMemcpyAsync(hx + a * n, p->x() + a * p->np_alloc, (size_t)n * sizeof(float),
+                               hipMemcpyDeviceToHost, stream()));
+    sync();
+    g_fof_done = false;
+    const float* xyz[3] = {hx, hx + n, hx + 2 * n};
+    fof_groups(xyz, n, (int64_t)p->np_alloc, p->boxsize, linking_length, boxsize3, quota, g_fof);
+    g_fof_done = true;
+    // the tree build reorders x v dx1 dx2 id (kdtree.cpp:94-105 sorts the
+    // particle records); the separate force array keeps its order
+    if (n > 0) {
+      int32_t* hperm = (int32_t*)(hx + 3 * n);
+      std::copy(g_fof.perm.begin(), g_fof.perm.end(), hperm);
+      DeviceTmp perm((size_t)n * sizeof(int32_t)), tmp((size_t)n * sizeof(uint64_t));
+      FS2_HIP(hipMemcpyAsync(perm.p, hperm, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
+                             stream()));
+      for (int k = 0; k < 4; ++k)
+        for (int a = 0; a < 3; ++a) {
+          uint32_t* plane = (uint32_t*)(p->field(k) + a * p->np_alloc);
+          gather_u32((uint32_t*)tmp.p, plane, (const int32_t*)perm.p, n);
+          FS2_HIP(hipMemcpyAsync(plane, tmp.p, (size_t)n * sizeof(float),
+                                 hipMemcpyDeviceToDevice, stream()));
+        }
+      gather_u64((uint64_t*)tmp.p, p->id, (const int32_t*)perm.p, n);
+      FS2_HIP(hipMemcpyAsync(p->id, tmp.p, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                             stream()));
+      sync();   // the temporaries and the pinned buffer are freed on return
+    }
+    touch_x(p);
+    if (ngroups) {
+      uint64_t c = 0;
+      for (int64_t i = 0; i < n; ++i) c += g_fof.grp[i] == (int32_t)i;
+      *ngroups = c;
+    }
+  });
+}
+
+int fs2_fof_grp(int32_t* out, uint64_t* n) {
+  return guard([&] {
+    if (n) *n = g_fof_done ? g_fof.grp.size() : 0;
+    if (out && g_fof_done) std::copy(g_fof.grp.begin(), g_fof.grp.end(), out);
+  });
+}
+
+int fs2_fof_nfof(int32_t* out, uint64_t* n) {
+  return guard([&] {
+    if (n) *n = g_fof_done ? g_fof.grp.size() : 0;
+    if (out && g_fof_done) {
+      std::vector<int32_t> c;
+      fof_counts(g_fof.grp, c);
+      std::copy(c.begin(), c.end(), out);
+    }
+  });
+}
+
+int fs2_kdtree_height(uint64_t* height) {
+  return guard([&] {
+    if (!g_fof_done) fail(FS2_ERR_RUNTIME, "no kd-tree: run fs2_fof_find_groups first");
+    *height = (uint64_t)g_fof.height;
+  });
+}
+
+int fs2_kdtree_nodes(fs2_kdtree_node* out, uint64_t* n) {
+  return guard([&] {
+    if (!g_fof_done) fail(FS2_ERR_RUNTIME, "no kd-tree: run fs2_fof_find_groups first");
+    if (n) *n = g_fof.tree.size();
+    if (out)
+      for (size_t i = 0; i < g_fof.tree.size(); ++i) {
+        const KdNode& t = g_fof.tree[i];
+        out[i] = fs2_kdtree_node{t.k, t.left, t.right, t.ibegin, t.iend};
+      }
+  });
+}
+
 // ---- output: Gadget-1 binary ---------------------------------------------------------------
 namespace {
 // py/gadget_file.h:6-25: the 256-byte Gadget-1 header (layout kept for the format)

@@ -71,6 +71,15 @@ __global__ void k_kick_drift(float* __restrict__ x, float* __restrict__ v,
   }
 }
 
+// dst[i] = src[perm[i]]: applies the kd-tree order of the FoF finder to a
+// particle plane (32-bit fields) or the id array (64-bit).
+template <typename T>
+__global__ void k_gather(T* __restrict__ dst, const T* __restrict__ src,
+                         const int32_t* __restrict__ perm, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
 // cola.cpp:141-145: v + Dv dx1 + D2v dx2 (Float coefficients).
 __global__ void k_velocity(float* __restrict__ out, const float* __restrict__ v,
                            const float* __restrict__ dx1, const float* __restrict__ dx2,
@@ -171,6 +180,18 @@ void cola_kick_drift(float* x, float* v, const float* f, const float* dx1, const
   if (np == 0) return;
   launch("cola_kick_drift", (double)np * 84.0, k_kick_drift, dim3(blocks_for(np)), dim3(kBlock),
          0, x, v, f, dx1, dx2, stride, np, m15om, kf, q1, q2, dt, da1, da2);
+}
+
+void gather_u32(uint32_t* dst, const uint32_t* src, const int32_t* perm, int64_t n) {
+  if (n == 0) return;
+  launch("gather_u32", (double)n * 12.0, k_gather<uint32_t>, dim3(blocks_for(n)), dim3(kBlock),
+         0, dst, src, perm, n);
+}
+
+void gather_u64(uint64_t* dst, const uint64_t* src, const int32_t* perm, int64_t n) {
+  if (n == 0) return;
+  launch("gather_u64", (double)n * 20.0, k_gather<uint64_t>, dim3(blocks_for(n)), dim3(kBlock),
+         0, dst, src, perm, n);
 }
 
 void cola_velocity(float* out, const float* v, const float* dx1, const float* dx2,

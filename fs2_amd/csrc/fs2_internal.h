@@ -206,6 +206,28 @@ void cola_kick(float* v, const float* f, const float* dx1, const float* dx2,
                int64_t stride, int64_t np, double m15om, float kf, float q1, float q2);
 void cola_drift(float* x, const float* v, const float* dx1, const float* dx2,
                 int64_t stride, int64_t np, double dt, float da1, float da2);
+// ---- friends-of-friends (host, fof.cpp) -------------------------------------
+// One kd-tree node (py/kdtree.h:9-15): children of node i are 2i+1, 2i+2.
+struct KdNode {
+  int k;               // cut axis of the parent (root: 0)
+  float left, right;   // extent of the node's particles along k
+  int32_t ibegin, iend;
+};
+struct FofResult {
+  std::vector<KdNode> tree;
+  int height = 0, quota = 0;
+  std::vector<int32_t> grp;    // lowest index of each particle's group
+  std::vector<int32_t> perm;   // index before the tree build, per final index
+};
+// Builds the tree over positions xyz[a][i] (i < n), reorders nothing in
+// place but returns the reference's particle order in perm, and links groups.
+void fof_groups(const float* const xyz[3], int64_t n, int64_t np_alloc, double boxsize,
+                double linking_length, const double* boxsize3, int quota, FofResult& out);
+void fof_counts(const std::vector<int32_t>& grp, std::vector<int32_t>& nfof);
+// dst[i] = src[perm[i]] for n elements of 4 or 8 bytes
+void gather_u32(uint32_t* dst, const uint32_t* src, const int32_t* perm, int64_t n);
+void gather_u64(uint64_t* dst, const uint64_t* src, const int32_t* perm, int64_t n);
+
 // kick then drift in one pass (bitwise equal to the two calls in sequence)
 void cola_kick_drift(float* x, float* v, const float* f, const float* dx1, const float* dx2,
                      int64_t stride, int64_t np, double m15om, float kf, float q1, float q2,

@@ -422,6 +422,82 @@ PyObject* py_particles_periodic_wrapup(PyObject*, PyObject* args) {
   Py_RETURN_NONE;
 }
 
+// ---- friends-of-friends (py/py_fof.cpp, py/py_kdtree.cpp) -------------------------------
+PyObject* int32_array_from(int (*get)(int32_t*, uint64_t*)) {
+  uint64_t n = 0;
+  CALL(get(nullptr, &n));
+  if (n == 0) Py_RETURN_NONE;
+  npy_intp dims[1] = {(npy_intp)n};
+  PyObject* arr = PyArray_SimpleNew(1, dims, NPY_INT32);
+  if (!arr) return nullptr;
+  const int rc = get((int32_t*)PyArray_DATA((PyArrayObject*)arr), &n);
+  if (rc) {
+    Py_DECREF(arr);
+    return raise(rc);
+  }
+  return arr;
+}
+
+// py_fof.cpp:14-54: _fof_find_groups(_particles, linking_length, boxsize3,
+// quota, return_nfof) -> nfof array or None
+PyObject* py_fof_find_groups(PyObject*, PyObject* args) {
+  PyObject *o, *py_box;
+  double ll;
+  int quota, return_nfof;
+  if (!PyArg_ParseTuple(args, "OdOii", &o, &ll, &py_box, &quota, &return_nfof)) return nullptr;
+  fs2_particles* p = as_particles(o);
+  PY_ASSERT_PTR(p);
+  double box[3];
+  const double* pbox = nullptr;
+  if (py_box != Py_None) {
+    PY_ASSERT_PTR(PySequence_Check(py_box));
+    for (int k = 0; k < 3; ++k) {
+      PyObject* e = PySequence_GetItem(py_box, k);
+      PY_ASSERT_PTR(e);
+      const bool is_float = PyFloat_Check(e);
+      box[k] = is_float ? PyFloat_AsDouble(e) : 0.0;
+      Py_DECREF(e);
+      PY_ASSERT_PTR(is_float);
+    }
+    pbox = box;
+  }
+  CALL(fs2_fof_find_groups(p, ll, pbox, quota, nullptr));
+  if (return_nfof) return int32_array_from(fs2_fof_nfof);
+  Py_RETURN_NONE;
+}
+
+// py_fof.cpp:56-64: _fof_grp() -> grp array, or None before any run
+PyObject* py_fof_grp(PyObject*, PyObject*) { return int32_array_from(fs2_fof_grp); }
+
+// py_kdtree.cpp:6-36: list of (i, k, left, right, ibegin, iend)
+PyObject* py_kdtree_create_copy(PyObject*, PyObject*) {
+  uint64_t n = 0;
+  CALL(fs2_kdtree_nodes(nullptr, &n));
+  std::vector<fs2_kdtree_node> nodes(n);
+  CALL(fs2_kdtree_nodes(nodes.data(), &n));
+  PyObject* list = PyList_New(0);
+  if (!list) return nullptr;
+  for (uint64_t i = 0; i < n; ++i) {
+    const fs2_kdtree_node& t = nodes[i];
+    PyObject* item = Py_BuildValue("(KiddKK)", (unsigned long long)i, (int)t.k, (double)t.left,
+                                   (double)t.right, (unsigned long long)t.ibegin,
+                                   (unsigned long long)t.iend);
+    if (!item || PyList_Append(list, item) != 0) {
+      Py_XDECREF(item);
+      Py_DECREF(list);
+      return nullptr;
+    }
+    Py_DECREF(item);
+  }
+  return list;
+}
+
+PyObject* py_kdtree_get_height(PyObject*, PyObject*) {
+  uint64_t h = 0;
+  CALL(fs2_kdtree_height(&h));
+  return Py_BuildValue("K", (unsigned long long)h);
+}
+
 PyObject* py_particles_clear(PyObject*, PyObject* args) {
   PyObject* o;
   if (!PyArg_ParseTuple(args, "O", &o)) return nullptr;
@@ -1058,6 +1134,11 @@ PyMethodDef methods[] = {
     M("_particles_v_asarray", py_particles_v_asarray, "new: velocities (np_local, 3)"),
     M("_particles_set_field", py_particles_set_field, "new: upload one (np_local,3) field"),
     M("_particles_periodic_wrapup", py_particles_periodic_wrapup, "periodic wrap of x"),
+    M("_fof_find_groups", py_fof_find_groups,
+      "_fof_find_groups(_particles, linking_length, boxsize3, quota, return_nfof)"),
+    M("_fof_grp", py_fof_grp, "_fof_grp()"),
+    M("_kdtree_create_copy", py_kdtree_create_copy, "_kdtree_create_copy()"),
+    M("_kdtree_get_height", py_kdtree_get_height, "_kdtree_get_height()"),
     M("_write_gadget_binary", py_write_gadget_binary,
       "_write_gadget_binary(_particles, filename, use_long_id)"),
     M("_hdf5_write_particles", py_hdf5_write_particles,

@@ -2,7 +2,7 @@
 
 Mirrors py/fs/__init__.py of the reference: importing the package
 initialises communication (one rank here) and exposes the same modules.
-FoF and the kd-tree are not part of this package.
+FoF and the kd-tree run on the host over positions copied from the device.
 """
 import atexit
 
@@ -13,10 +13,12 @@ from . import lpt
 from . import pm
 from . import cola
 from . import leapfrog
+from . import fof
 from . import msg
 from .particles import Particles
 from .power import PowerSpectrum
 from .fft import FFT
+from .kdtree import KdTree
 
 _fs.comm_mpi_init()
 

@@ -100,6 +100,12 @@ class Particles(object):
         return c._particles_force_asarray(self._particles)
 
     @property
+    def fof_group(self):
+        """fof_group[i] is the group (its lowest particle index) of particle i
+        after the last fs.fof.find_groups; None before any run."""
+        return c._fof_grp()
+
+    @property
     def times(self):
         """(a_x, a_v, a_f)"""
         return c._particles_times(self._particles)

@@ -232,6 +232,36 @@ int fs2_cola_drift(fs2_particles* p, double a_pos);
 /* py/cola.cpp:129-148 cola_velocity: (np_local, 3) FP32 into out */
 int fs2_cola_velocity(fs2_particles* p, float* out);
 
+/* ---- friends-of-friends halo finder (CPU, fed by a pinned D2H) -------- */
+/* py/py_fof.cpp:14-54 _fof_find_groups / py/fof.cpp:45-99 fof_find_groups:
+ * copies the np_local positions to pinned host memory, builds the balanced
+ * kd-tree (py/kdtree.cpp:119-190) and links pairs closer than
+ * linking_length (periodic, FP32).  boxsize3 (3 doubles, or NULL for the
+ * bounding box) only steers the cut directions.  As in the reference the
+ * tree build reorders the particles: x, v, dx1, dx2 and id are permuted on
+ * the device, the force array is not.  ngroups (optional) receives the
+ * number of groups.  The result stays in library state for the calls below. */
+int fs2_fof_find_groups(fs2_particles* p, double linking_length,
+                        const double* boxsize3, int quota, uint64_t* ngroups);
+/* py/py_fof.cpp:56-64 _fof_grp (py/fof.cpp:121-124): grp[i] = lowest index
+ * of particle i's group.  *n = np_local of the last run (0: none); out may
+ * be NULL to query n. */
+int fs2_fof_grp(int32_t* out, uint64_t* n);
+/* py/fof.cpp:101-113 fof_compute_nfof: nfof[i] = members of the group whose
+ * lowest index is i, else 0.  Same n convention as fs2_fof_grp. */
+int fs2_fof_nfof(int32_t* out, uint64_t* n);
+/* py/kdtree.h:9-15 KdTree node; py/py_kdtree.cpp:6-36 _kdtree_create_copy
+ * and _kdtree_get_height (py/kdtree.cpp:212-220).  Node i has children
+ * 2i+1 and 2i+2; there are 2^(height+1) - 1 nodes, sized from the
+ * allocation, and the ones below the leaves are zero. */
+typedef struct fs2_kdtree_node {
+  int32_t k;
+  float left, right;
+  int32_t ibegin, iend;
+} fs2_kdtree_node;
+int fs2_kdtree_height(uint64_t* height);
+int fs2_kdtree_nodes(fs2_kdtree_node* out, uint64_t* n);
+
 /* ---- output ------------------------------------------------------------------ */
 /* py/gadget_file.cpp:15-114 / py_write.cpp _write_gadget_binary: Gadget-1
  * binary snapshot (256-byte header, positions, velocities / sqrt(a),
