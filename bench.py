@@ -137,6 +137,38 @@ def batch_leg(fs, ps, reps=5):
     }
 
 
+def fof_leg(fs, ps, n_p=256, nc=512, box=512.0, nstep=10):
+    """Friends-of-friends on the final positions of a COLA run (n_p^3
+    particles, nc^3 mesh, 10 steps a=0.1->1), linking length 0.2 x mean
+    separation.  Times the whole call: D2H of x through pinned memory, the
+    kd-tree build and the linking on the host threads (OMP_NUM_THREADS, else
+    all cores, at most 64), and the device reordering."""
+    import numpy as np
+    p = fs.lpt.init(n_p, box, 0.1, ps, 1, "cola")
+    fs.pm.init(nc, nc / n_p, box)
+    fs.pm.set_paint_variant("sorted")
+    for a_vel, a_pos in schedule(0.1, 1.0, nstep):
+        cola_step(fs, p, a_vel, a_pos)
+    fs._fs._synchronize()
+    ll = 0.2 * box / n_p
+    t0 = time.perf_counter()
+    nfof = fs.fof.find_groups(p, ll, compute_nfof=True)
+    t = time.perf_counter() - t0
+    sizes = nfof[nfof > 0]
+    return {
+        "workload": f"FoF b=0.2 on {n_p}^3 particles after 10 COLA steps ({nc}^3 mesh, box {box:g})",
+        "particles": n_p**3,
+        "seconds": round(t, 4),
+        "value": n_p**3 / t,
+        "unit": "particles/s",
+        "host_threads": max(1, min(64, int(os.environ.get("OMP_NUM_THREADS", "0")) or
+                                   (os.cpu_count() or 1))),
+        "groups": int(len(sizes)),
+        "groups_ge_20": int(np.count_nonzero(sizes >= 20)),
+        "largest": int(sizes.max()),
+    }
+
+
 def cpu_baseline(n_p, nc, box, sub, workers):
     """Oracle port on the host: one COLA step with the four FFTs at full mesh
     size (FP32 scipy.fft, `workers` threads) and the particle loops (paint,
@@ -323,7 +355,8 @@ def main():
     ap.add_argument("--cpu-sub", type=int, default=8)
     ap.add_argument("--legs", default="main,adjoint,batch",
                     help="comma list of main (headline COLA box), adjoint (differentiable "
-                         "128^3/256^3 run), batch (96-box hipGraph ensemble)")
+                         "128^3/256^3 run), batch (96-box hipGraph ensemble), fof "
+                         "(halo finder after a 256^3 COLA run; not in the default list)")
     args = ap.parse_args()
     legs = set(x.strip() for x in args.legs.split(",") if x.strip())
 
@@ -356,6 +389,8 @@ def main():
         "batch": None,
         "cpu_baseline": None,
     }
+    if "fof" in legs:
+        out["fof"] = None
     if "main" in legs:
         out.update(main_leg(args, fs, ps))
         out["copy_peak"] = copy_probe()
@@ -367,6 +402,9 @@ def main():
     if "batch" in legs:
         print("[bench] batched ensemble (96 boxes, hipGraph)", file=sys.stderr, flush=True)
         out["batch"] = batch_leg(fs, ps)
+    if "fof" in legs:
+        print("[bench] friends-of-friends 256^3", file=sys.stderr, flush=True)
+        out["fof"] = fof_leg(fs, ps)
     if args.cpu_baseline == "on" and "main" in legs:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         try:
