@@ -157,9 +157,8 @@ void run_schedule(fs2_batch* b) {
     StreamScope scope(c.s);
     for (const StepCoef& s : coef) {
       class_force(c);
-      cola_kick(c.field(1), c.field(4), c.field(2), c.field(3), c.np, c.np, m15om, s.kf, s.q1,
-                s.q2);
-      cola_drift(c.field(0), c.field(1), c.field(2), c.field(3), c.np, c.np, s.dt, s.da1, s.da2);
+      cola_kick_drift(c.field(0), c.field(0), c.field(1), c.field(4), c.field(2), c.field(3), c.np,
+                      c.np, m15om, s.kf, s.q1, s.q2, s.dt, s.da1, s.da2);
     }
     FS2_HIP(hipEventRecord(b->join[k], c.s));
   }

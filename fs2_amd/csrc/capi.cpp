@@ -929,7 +929,7 @@ int fs2_cola_drift(fs2_particles* p, double a_pos) {
     // cola.cpp:106-109: da1, da2 stored as Float; dt stays double
     if (p->kick.pending) {
       p->kick.pending = false;
-      cola_kick_drift(p->x(), p->v(), p->f(), p->dx1(), p->dx2(), (int64_t)p->np_alloc,
+      cola_kick_drift(p->x(), p->x(), p->v(), p->f(), p->dx1(), p->dx2(), (int64_t)p->np_alloc,
                       (int64_t)p->np_local, p->kick.m15om, p->kick.kf, p->kick.q1, p->kick.q2,
                       dt, (float)da1, (float)da2);
     } else {

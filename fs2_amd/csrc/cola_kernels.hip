@@ -49,8 +49,9 @@ __global__ void k_drift(float* __restrict__ x, const float* __restrict__ v,
 // k_kick followed by k_drift on the same particle, fused: v is written by
 // the kick expression and that rounded FP32 value feeds the drift, exactly
 // as when the two kernels run in sequence.  One pass reads x v f dx1 dx2
-// and writes x v.
-__global__ void k_kick_drift(float* __restrict__ x, float* __restrict__ v,
+// and writes x v.  The drifted positions go to x, read from xin (the same
+// array, or the previous checkpoint of a differentiable run).
+__global__ void k_kick_drift(float* x, const float* xin, float* __restrict__ v,
                              const float* __restrict__ f, const float* __restrict__ dx1,
                              const float* __restrict__ dx2, int64_t stride, int64_t np,
                              double m15om, float kf, float q1, float q2, double dt, float da1,
@@ -67,7 +68,7 @@ __global__ void k_kick_drift(float* __restrict__ x, float* __restrict__ v,
     v[j] = vn;
     const float lpt = d1 * da1 + d2 * da2;
     const double rhs = (double)vn * dt + (double)lpt;
-    x[j] = (float)((double)x[j] + rhs);
+    x[j] = (float)((double)xin[j] + rhs);
   }
 }
 
@@ -174,12 +175,12 @@ void cola_drift(float* x, const float* v, const float* dx1, const float* dx2, in
          dx1, dx2, stride, np, dt, da1, da2);
 }
 
-void cola_kick_drift(float* x, float* v, const float* f, const float* dx1, const float* dx2,
-                     int64_t stride, int64_t np, double m15om, float kf, float q1, float q2,
-                     double dt, float da1, float da2) {
+void cola_kick_drift(float* x, const float* xin, float* v, const float* f, const float* dx1,
+                     const float* dx2, int64_t stride, int64_t np, double m15om, float kf,
+                     float q1, float q2, double dt, float da1, float da2) {
   if (np == 0) return;
   launch("cola_kick_drift", (double)np * 84.0, k_kick_drift, dim3(blocks_for(np)), dim3(kBlock),
-         0, x, v, f, dx1, dx2, stride, np, m15om, kf, q1, q2, dt, da1, da2);
+         0, x, xin, v, f, dx1, dx2, stride, np, m15om, kf, q1, q2, dt, da1, da2);
 }
 
 void gather_u32(uint32_t* dst, const uint32_t* src, const int32_t* perm, int64_t n) {

@@ -300,11 +300,9 @@ int fs2_diff_forward(fs2_diff* d, const float* noise, float* delta) {
         readout_tiled(F[0], F[1], F[2], d->f, np, np, nm, d->boxsize_f, d->bins_of(i), d->tile);
       else
         readout3(PosArgs{d->x_at(i), np, np}, F[0], F[1], F[2], d->f, nm, d->boxsize_f);
-      cola_kick(d->v, d->f, d->dx1, d->dx2, np, np, m15om, s.kf, s.q1, s.q2);
-      float* xn = d->x_at(i + 1);
-      FS2_HIP(hipMemcpyAsync(xn, d->x_at(i), 3 * (size_t)np * sizeof(float),
-                             hipMemcpyDeviceToDevice, stream()));
-      cola_drift(xn, d->v, d->dx1, d->dx2, np, np, s.dt, s.da1, s.da2);
+      // kick, then drift from checkpoint i into checkpoint i + 1
+      cola_kick_drift(d->x_at(i + 1), d->x_at(i), d->v, d->f, d->dx1, d->dx2, np, np, m15om,
+                      s.kf, s.q1, s.q2, s.dt, s.da1, s.da2);
     }
     density(d, d->nstep, d->paint, true);
     from_grid(delta, d->mesh, nm);

@@ -228,10 +228,11 @@ void fof_counts(const std::vector<int32_t>& grp, std::vector<int32_t>& nfof);
 void gather_u32(uint32_t* dst, const uint32_t* src, const int32_t* perm, int64_t n);
 void gather_u64(uint64_t* dst, const uint64_t* src, const int32_t* perm, int64_t n);
 
-// kick then drift in one pass (bitwise equal to the two calls in sequence)
-void cola_kick_drift(float* x, float* v, const float* f, const float* dx1, const float* dx2,
-                     int64_t stride, int64_t np, double m15om, float kf, float q1, float q2,
-                     double dt, float da1, float da2);
+// kick then drift in one pass (bitwise equal to the two calls in sequence);
+// the drift reads positions from xin and writes them to x (may be equal)
+void cola_kick_drift(float* x, const float* xin, float* v, const float* f, const float* dx1,
+                     const float* dx2, int64_t stride, int64_t np, double m15om, float kf,
+                     float q1, float q2, double dt, float da1, float da2);
 void cola_velocity(float* out, const float* v, const float* dx1, const float* dx2,
                    int64_t stride, int64_t np, float Dv, float D2v);
 void scale_field(float* out, const float* in, int64_t stride, int64_t np, float s);
