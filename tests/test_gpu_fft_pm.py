@@ -47,6 +47,49 @@ def test_c2r_non_hermitian_matches_irfftn(fs, n):
     assert err < 2e-6, err
 
 
+@pytest.mark.parametrize("n", [256, 512, 1024])
+def test_r2c_separable_closed_form(fs, n):
+    """Large meshes (the persistent z passes run from n = 512): the input is
+    a sum of two separable terms a(x) b(y) c(z), whose transform is the sum
+    of the products of 1D transforms A(kx) B(ky) C(kz), compared slab by slab."""
+    _announce(f"r2c separable n={n}")
+    rng = np.random.default_rng(7 * n)
+    terms = [[rng.standard_normal(n) for _ in range(3)] for _ in range(2)]
+    a = np.empty((n, n, n), dtype=np.float32)
+    for i in range(n):   # slab by slab: no n^3 FP64 temporaries
+        a[i] = sum((ta[i] * tb[:, None] * tc[None, :]).astype(np.float32) for ta, tb, tc in terms)
+    # what the device sees is the FP32 sum; its transform is that of the
+    # FP64 terms up to the input rounding, bounded below with the tolerance
+    g = fs.FFT(n)
+    g.set_array(a)
+    del a
+    g.execute_forward()
+    got = g.asarray_k()
+    spec = [(np.fft.fft(ta), np.fft.fft(tb), np.fft.rfft(tc)) for ta, tb, tc in terms]
+    scale = max(np.abs(A).max() * np.abs(B).max() * np.abs(C).max() for A, B, C in spec)
+    err = 0.0
+    for kx in range(n):
+        ref = sum(A[kx] * B[:, None] * C[None, :] for A, B, C in spec)
+        err = max(err, float(np.abs(got[kx] - ref).max()))
+    assert err / scale < 4e-6, err / scale
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+def test_round_trip_large(fs, n):
+    """c2r(r2c(x)) = n^3 x at the sizes of the persistent z passes."""
+    _announce(f"round trip n={n}")
+    rng = np.random.default_rng(n + 3)
+    a = rng.standard_normal((n, n, n), dtype=np.float32)
+    g = fs.FFT(n)
+    g.set_array(a)
+    g.execute_forward()
+    g.execute_inverse()
+    back = g.asarray()
+    back /= np.float32(n) ** 3
+    err = float(np.abs(back - a).max()) / float(np.abs(a).max())
+    assert err < 2e-6, err
+
+
 def test_fft_test_data_layout(fs):
     """test/test_fft.py:9-28: set_test_data then gather -> index + 1."""
     _announce("fft test data")
