@@ -236,11 +236,19 @@ def measured_traffic(kernel):
     return None
 
 
-def copy_probe(gib=4):
-    """Device-to-device copy of `gib` GiB (torch's copy kernel), timed with
-    events: the practical ceiling of a 1-read + 1-write streaming pass."""
+def copy_probe(fs, gib=4):
+    """The practical ceiling of a 1-read + 1-write streaming pass: the best
+    of the library's grid-stride float4 copy over a few grid sizes
+    (`fs2_copy_bandwidth`, HIP events on the library stream), with torch's
+    copy kernel beside it.  `gib` GiB are read and as many written."""
     import torch
-    n = gib * (1 << 30) // 4
+    nbytes = gib * (1 << 30)
+    grids = {}
+    for g in (1024, 2048, 4096, 16384, 65536):
+        ms = fs._fs._copy_bandwidth(nbytes, g, 5)
+        grids[g] = round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
+    g_best = max(grids, key=grids.get)
+    n = nbytes // 4
     a = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
     b = torch.empty_like(a)
     b.copy_(a)
@@ -252,9 +260,11 @@ def copy_probe(gib=4):
         b.copy_(a)
     e1.record()
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+    ms_t = e0.elapsed_time(e1) / reps
     del a, b
-    return {"bytes": 2 * n * 4, "ms": round(ms, 4), "GBps": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
+    return {"bytes": 2 * nbytes, "GBps": grids[g_best], "kernel": "grid-stride float4 copy",
+            "grid": g_best, "GBps_by_grid": grids,
+            "torch_copy_GBps": round(2 * nbytes / (ms_t * 1e-3) / 1e9, 1)}
 
 
 def main_leg(args, fs, ps):
@@ -393,7 +403,7 @@ def main():
         out["fof"] = None
     if "main" in legs:
         out.update(main_leg(args, fs, ps))
-        out["copy_peak"] = copy_probe()
+        out["copy_peak"] = copy_probe(fs)
         out["roofline"]["frac_of_copy_peak"] = round(
             out["roofline"]["achieved"] / out["copy_peak"]["GBps"], 4)
     if "adjoint" in legs:

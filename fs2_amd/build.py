@@ -38,7 +38,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ['fft.hip', 'pm_kernels.hip', 'paint_tiled.hip', 'cola_kernels.hip', 'lpt_kernels.hip',
-           'adjoint_kernels.hip', 'capi.cpp', 'diff.cpp', 'batch.cpp', 'host_numerics.cpp', 'fof.cpp']
+           'adjoint_kernels.hip', 'capi.cpp', 'diff.cpp', 'batch.cpp', 'host_numerics.cpp', 'fof.cpp', 'probe.hip']
 
 
 def _run(cmd):

@@ -1306,6 +1306,14 @@ int fs2_lpt(int nc, double boxsize, double a, uint64_t seed, const fs2_power* ps
   });
 }
 
+// ---- streaming-copy probe (roofline harness) ----------------------------------------------
+int fs2_copy_bandwidth(uint64_t bytes, int grid, int reps, double* ms) {
+  return guard([&] {
+    if (!ms) fail(FS2_ERR_VALUE, "null output");
+    *ms = copy_probe_ms(bytes, grid, reps);
+  });
+}
+
 // ---- kernel timing ---------------------------------------------------------------------------------------------
 int fs2_profile_enable(int on) {
   return guard([&] {

@@ -206,6 +206,9 @@ void cola_kick(float* v, const float* f, const float* dx1, const float* dx2,
                int64_t stride, int64_t np, double m15om, float kf, float q1, float q2);
 void cola_drift(float* x, const float* v, const float* dx1, const float* dx2,
                 int64_t stride, int64_t np, double dt, float da1, float da2);
+// Best time (ms) of a grid-stride float4 copy of `bytes` (probe.hip).
+double copy_probe_ms(uint64_t bytes, int grid, int reps);
+
 // ---- friends-of-friends (host, fof.cpp) -------------------------------------
 // One kd-tree node (py/kdtree.h:9-15): children of node i are 2i+1, 2i+2.
 struct KdNode {

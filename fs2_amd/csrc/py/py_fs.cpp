@@ -1080,6 +1080,16 @@ PyObject* py_profile_reset(PyObject*, PyObject*) {
   CALL(fs2_profile_reset());
   Py_RETURN_NONE;
 }
+// new: _copy_bandwidth(bytes, grid, reps) -> best ms of a streaming float4 copy
+PyObject* py_copy_bandwidth(PyObject*, PyObject* args) {
+  unsigned long long bytes;
+  int grid, reps;
+  if (!PyArg_ParseTuple(args, "Kii", &bytes, &grid, &reps)) return nullptr;
+  double ms = 0.0;
+  CALL(fs2_copy_bandwidth(bytes, grid, reps, &ms));
+  return PyFloat_FromDouble(ms);
+}
+
 PyObject* py_profile_get(PyObject*, PyObject*) {
   int n;
   CALL(fs2_profile_count(&n));
@@ -1187,6 +1197,7 @@ PyMethodDef methods[] = {
     M("_synchronize", py_synchronize, "new: wait for device work"),
     M("_profile_enable", py_profile_enable, "new: per-kernel HIP-event timing"),
     M("_profile_reset", py_profile_reset, "new"),
+    M("_copy_bandwidth", py_copy_bandwidth, "new: best ms of a streaming float4 copy"),
     M("_profile_get", py_profile_get, "new: [(name, launches, total_ms, bytes_per_launch)]"),
     {nullptr, nullptr, 0, nullptr}};
 

@@ -232,6 +232,12 @@ int fs2_cola_drift(fs2_particles* p, double a_pos);
 /* py/cola.cpp:129-148 cola_velocity: (np_local, 3) FP32 into out */
 int fs2_cola_velocity(fs2_particles* p, float* out);
 
+/* ---- measurement --------------------------------------------------------- */
+/* No reference counterpart (roofline harness): best time in ms over reps
+ * launches of a grid-stride float4 copy of `bytes` between two fresh device
+ * buffers, `grid` blocks of 256 threads, on the library stream. */
+int fs2_copy_bandwidth(uint64_t bytes, int grid, int reps, double* ms);
+
 /* ---- friends-of-friends halo finder (CPU, fed by a pinned D2H) -------- */
 /* py/py_fof.cpp:14-54 _fof_find_groups / py/fof.cpp:45-99 fof_find_groups:
  * copies the np_local positions to pinned host memory, builds the balanced
