@@ -37,22 +37,28 @@ def cola_step(fs, p, a_vel, a_pos):
     fs.cola.drift(p, a_pos)
 
 
-def adjoint_leg(fs, ps, reps=5):
-    """Differentiable COLA: 128^3 particles, 256^3 mesh, box 256 Mpc/h, 10 steps
-    a = 0.1 -> 1; loss = mean((delta - target)^2) against the final density of
-    a second seed.  Times the checkpointing forward and the adjoint sweep
-    (gradient with respect to the 128^3 white noise), both device-resident,
-    for the two checkpoint policies: 'store' (every step's force meshes and
-    bins kept; the default whenever it fits, 2.6 GB here) and 'recompute'
+def adjoint_leg(fs, ps, reps=5, n=128, nm=256, L=256.0):
+    """Differentiable COLA: n^3 particles, nm^3 mesh (default 128^3 / 256^3,
+    box 256 Mpc/h), 10 steps a = 0.1 -> 1; loss = mean((delta - target)^2)
+    against the final density of a second seed.  Times the checkpointing
+    forward and the adjoint sweep (gradient with respect to the n^3 white
+    noise), both device-resident, for the two checkpoint policies: 'store'
+    (every step's force meshes and bins kept; the default whenever it fits,
+    2.6 GB at 128^3 / 256^3, 154 GB at 512^3 / 1024^3) and 'recompute'
     (positions only)."""
     import torch
     from fs2_amd.autograd import ColaModel
     c = fs._fs
-    n, nm, L, nstep = 128, 256, 256.0, 10
+    nstep = 10
     res = {"workload": f"differentiable COLA {n}^3 particles, {nm}^3 mesh, box {L:g} Mpc/h, "
                        f"{nstep} steps a=0.1->1, d MSE(delta, target) / d noise"}
     for policy in ("store", "recompute"):
-        model = ColaModel(n, nm, L, 0.1, 1.0, nstep, ps, checkpoint=policy)
+        print(f"[bench] adjoint {n}^3/{nm}^3 {policy}", file=sys.stderr, flush=True)
+        try:
+            model = ColaModel(n, nm, L, 0.1, 1.0, nstep, ps, checkpoint=policy)
+        except MemoryError as e:   # 'store' at large sizes may not fit
+            res[policy] = {"error": str(e)}
+            continue
         gen = torch.Generator(device="cuda").manual_seed(2)
         target = model.forward(torch.randn((n,) * 3, device="cuda", generator=gen))
         gen.manual_seed(1)
@@ -85,7 +91,8 @@ def adjoint_leg(fs, ps, reps=5):
                                         for name, _, total, _ in top},
         }
         del model
-    res["adjoint_forward_ratio"] = res["store"]["adjoint_forward_ratio"]
+    res["adjoint_forward_ratio"] = res["store"].get("adjoint_forward_ratio",
+                                                    res["recompute"]["adjoint_forward_ratio"])
     res["checkpoint_bytes"] = {"positions": 12 * n**3 * (nstep + 1),
                                "store_extra": nstep * (3 * 4 * nm * nm * 2 * ((nm // 2 + 1 + 15) // 16 * 16)
                                                        + 16 * n**3)}
@@ -366,7 +373,8 @@ def main():
     ap.add_argument("--legs", default="main,adjoint,batch",
                     help="comma list of main (headline COLA box), adjoint (differentiable "
                          "128^3/256^3 run), batch (96-box hipGraph ensemble), fof "
-                         "(halo finder after a 256^3 COLA run; not in the default list)")
+                         "(halo finder after a 256^3 COLA run), adjoint_c (the differentiable run at "
+                         "512^3/1024^3); fof and adjoint_c are not in the default list")
     args = ap.parse_args()
     legs = set(x.strip() for x in args.legs.split(",") if x.strip())
 
@@ -409,6 +417,8 @@ def main():
     if "adjoint" in legs:
         print("[bench] adjoint 128^3/256^3", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(fs, ps)
+    if "adjoint_c" in legs:
+        out["adjoint_512p_1024mesh"] = adjoint_leg(fs, ps, reps=2, n=512, nm=1024, L=1024.0)
     if "batch" in legs:
         print("[bench] batched ensemble (96 boxes, hipGraph)", file=sys.stderr, flush=True)
         out["batch"] = batch_leg(fs, ps)
