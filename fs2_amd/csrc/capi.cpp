@@ -1,6 +1,7 @@
 // C ABI of libfs2hip: handles, error codes, device buffers and the host-side
 // orchestration of the PM / COLA / 2LPT hot path.  See include/fs2hip.h.
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1304,6 +1305,44 @@ int fs2_lpt(int nc, double boxsize, double a, uint64_t seed, const fs2_power* ps
     cleanup();
     *out = p;
   });
+}
+
+// ---- named device allocations (py/mem.cpp, py/py_mem.cpp) -------------------------------------
+struct fs2_mem {
+  std::string name;
+  void* buf = nullptr;
+  uint64_t size_alloc = 0, size_using = 0;
+};
+
+int fs2_mem_alloc(const char* name, uint64_t size, fs2_mem** out) {
+  return guard([&] {
+    if (!out) fail(FS2_ERR_VALUE, "null output");
+    auto m = std::make_unique<fs2_mem>();
+    m->name = name ? name : "";
+    if (size > 0) {
+      const uint64_t aligned = (size + 15) / 16 * 16;   // mem.cpp:48-50, ALGN 16
+      m->buf = dmalloc((size_t)aligned);
+      m->size_alloc = aligned;
+    }
+    *out = m.release();
+  });
+}
+
+int fs2_mem_info(const fs2_mem* m, uint64_t* size_alloc, uint64_t* size_using) {
+  return guard([&] {
+    if (!m) fail(FS2_ERR_VALUE, "null mem handle");
+    if (size_alloc) *size_alloc = m->size_alloc;
+    if (size_using) *size_using = m->size_using;
+  });
+}
+
+void fs2_mem_free(fs2_mem* m) {
+  if (!m) return;
+  if (m->buf) {
+    if (g_inited) (void)hipStreamSynchronize(g_stream);
+    dfree(m->buf);
+  }
+  delete m;
 }
 
 // ---- streaming-copy probe (roofline harness) ----------------------------------------------

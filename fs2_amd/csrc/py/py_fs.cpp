@@ -1080,6 +1080,30 @@ PyObject* py_profile_reset(PyObject*, PyObject*) {
   CALL(fs2_profile_reset());
   Py_RETURN_NONE;
 }
+// ---- py_mem.cpp: _mem_alloc(name, size) -> "_Mem" capsule, _mem(m) -> (alloc, using)
+void mem_free(PyObject* cap) {
+  fs2_mem_free((fs2_mem*)PyCapsule_GetPointer(cap, "_Mem"));
+}
+
+PyObject* py_mem_alloc(PyObject*, PyObject* args) {
+  const char* name;
+  unsigned long long size;
+  if (!PyArg_ParseTuple(args, "sK", &name, &size)) return nullptr;
+  fs2_mem* m = nullptr;
+  CALL(fs2_mem_alloc(name, size, &m));
+  return PyCapsule_New(m, "_Mem", mem_free);
+}
+
+PyObject* py_mem(PyObject*, PyObject* args) {
+  PyObject* o;
+  if (!PyArg_ParseTuple(args, "O", &o)) return nullptr;
+  fs2_mem* m = (fs2_mem*)PyCapsule_GetPointer(o, "_Mem");
+  PY_ASSERT_PTR(m);
+  uint64_t a = 0, u = 0;
+  CALL(fs2_mem_info(m, &a, &u));
+  return Py_BuildValue("(KK)", (unsigned long long)a, (unsigned long long)u);
+}
+
 // new: _copy_bandwidth(bytes, grid, reps) -> best ms of a streaming float4 copy
 PyObject* py_copy_bandwidth(PyObject*, PyObject* args) {
   unsigned long long bytes;
@@ -1198,6 +1222,8 @@ PyMethodDef methods[] = {
     M("_profile_enable", py_profile_enable, "new: per-kernel HIP-event timing"),
     M("_profile_reset", py_profile_reset, "new"),
     M("_copy_bandwidth", py_copy_bandwidth, "new: best ms of a streaming float4 copy"),
+    M("_mem_alloc", py_mem_alloc, "create a new Mem object"),
+    M("_mem", py_mem, "return mem allocated and using"),
     M("_profile_get", py_profile_get, "new: [(name, launches, total_ms, bytes_per_launch)]"),
     {nullptr, nullptr, 0, nullptr}};
 

@@ -232,6 +232,16 @@ int fs2_cola_drift(fs2_particles* p, double a_pos);
 /* py/cola.cpp:129-148 cola_velocity: (np_local, 3) FP32 into out */
 int fs2_cola_velocity(fs2_particles* p, float* out);
 
+/* ---- named allocations ------------------------------------------------- */
+/* py/py_mem.cpp:8-31 _mem_alloc / py/mem.cpp:28-75 Mem(name, size): `size`
+ * bytes (rounded up to 16) of device memory, nothing for size 0;
+ * FS2_ERR_MEMORY when it does not fit. */
+typedef struct fs2_mem fs2_mem;
+int fs2_mem_alloc(const char* name, uint64_t size, fs2_mem** out);
+/* py/py_mem.cpp:42-53 _mem: (size_alloc, size_using) */
+int fs2_mem_info(const fs2_mem* m, uint64_t* size_alloc, uint64_t* size_using);
+void fs2_mem_free(fs2_mem* m);
+
 /* ---- measurement --------------------------------------------------------- */
 /* No reference counterpart (roofline harness): best time in ms over reps
  * launches of a grid-stride float4 copy of `bytes` between two fresh device

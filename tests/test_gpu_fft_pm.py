@@ -281,3 +281,16 @@ def test_tiled_readout_matches_gather_readout(fs, n_p, nc):
     ref = opm.force(p.x.astype(np.float64), nc, L, nc / n_p)
     rel = np.sqrt(np.mean((f_tile - ref) ** 2) / np.mean(ref**2))
     assert rel < 1e-5, rel
+
+
+def test_mem_named_allocation(fs):
+    """fs.mem.Mem (py/fs/mem.py): sizes rounded up to 16 bytes (mem.cpp:48-50),
+    nothing allocated for size 0, MemoryError when the device cannot hold it."""
+    _announce("mem")
+    from fs2_amd.fs.mem import Mem
+    m = Mem("test", 100)
+    assert (m.mem_allocated, m.mem_using) == (112, 0)
+    assert "test" in repr(m)
+    assert (Mem("empty").mem_allocated, Mem("empty").mem_using) == (0, 0)
+    with pytest.raises(MemoryError):
+        Mem("too big", 1 << 50)
