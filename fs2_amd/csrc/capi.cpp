@@ -547,6 +547,19 @@ int fs2_particles_append(fs2_particles* p, const double* x, uint64_t n, int64_t 
   });
 }
 
+int fs2_particles_resize(fs2_particles* p, uint64_t n) {
+  return guard([&] {
+    check_p(p);
+    if (n > p->np_alloc)
+      fail(FS2_ERR_MEMORY, "resize to %llu particles exceeds np_alloc %llu", (unsigned long long)n,
+           (unsigned long long)p->np_alloc);
+    flush_wrap(p);
+    p->np_local = n;
+    p->np_total = n;
+    touch_x(p);
+  });
+}
+
 int fs2_particles_clear(fs2_particles* p) {
   return guard([&] {
     check_p(p);

@@ -498,6 +498,17 @@ PyObject* py_kdtree_get_height(PyObject*, PyObject*) {
   return Py_BuildValue("K", (unsigned long long)h);
 }
 
+// new: _particles_resize(_particles, np_local)
+PyObject* py_particles_resize(PyObject*, PyObject* args) {
+  PyObject* o;
+  unsigned long long n;
+  if (!PyArg_ParseTuple(args, "OK", &o, &n)) return nullptr;
+  fs2_particles* p = as_particles(o);
+  PY_ASSERT_PTR(p);
+  CALL(fs2_particles_resize(p, n));
+  Py_RETURN_NONE;
+}
+
 PyObject* py_particles_clear(PyObject*, PyObject* args) {
   PyObject* o;
   if (!PyArg_ParseTuple(args, "O", &o)) return nullptr;
@@ -1168,6 +1179,7 @@ PyMethodDef methods[] = {
     M("_particles_v_asarray", py_particles_v_asarray, "new: velocities (np_local, 3)"),
     M("_particles_set_field", py_particles_set_field, "new: upload one (np_local,3) field"),
     M("_particles_periodic_wrapup", py_particles_periodic_wrapup, "periodic wrap of x"),
+    M("_particles_resize", py_particles_resize, "new: set np_local"),
     M("_fof_find_groups", py_fof_find_groups,
       "_fof_find_groups(_particles, linking_length, boxsize3, quota, return_nfof)"),
     M("_fof_grp", py_fof_grp, "_fof_grp()"),

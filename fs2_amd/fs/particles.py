@@ -25,6 +25,10 @@ class Particles(object):
     def update_np_total(self):
         c._particles_update_np_total(self._particles)
 
+    def resize(self, np_local):
+        """Set the local number of particles to np_local (<= the allocation)."""
+        c._particles_resize(self._particles, np_local)
+
     def append(self, x):
         """Append particles with positions x ((n, >=3) float64; None: no-op)."""
         c._particles_append(self._particles, x)

@@ -90,3 +90,11 @@ def power_spectrum(nbins=None, dk_bin=1.0, nc=None):
             raise ValueError('give nbins or nc')
         nbins = int(nc // 2 / dk_bin)
     return c._pm_power_spectrum(nbins, dk_bin)
+
+
+def write_packet_info(filename):
+    """MPI particle-exchange information (py/fs/pm.py:108-114).  The
+    reference's wrapper calls a method its module does not register, and
+    there are no packets on one rank; this raises as the reference does."""
+    raise AttributeError("_pm_write_packet_info is not available: particle packets exist only "
+                         "in multi-rank runs")

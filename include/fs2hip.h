@@ -145,6 +145,10 @@ int fs2_particles_append(fs2_particles* p, const double* x, uint64_t n,
                          int64_t row_stride, int64_t col_stride);
 /* py/py_particles.cpp:386-401 _particles_clear */
 int fs2_particles_clear(fs2_particles* p);
+/* py/fs/particles.py:61-63 Particles.resize (its _particles_reseize is not
+ * registered in the reference): np_local = n <= np_alloc; particles past the
+ * old np_local keep whatever the planes hold. */
+int fs2_particles_resize(fs2_particles* p, uint64_t n);
 /* py/py_particles.cpp:306-355 _particles_x_asarray (x, dx1, dx2) and
  * _particles_force_asarray: (np_local, 3) FP32 row-major into out. */
 int fs2_particles_get_field(fs2_particles* p, int field, float* out);

@@ -294,3 +294,18 @@ def test_mem_named_allocation(fs):
     assert (Mem("empty").mem_allocated, Mem("empty").mem_using) == (0, 0)
     with pytest.raises(MemoryError):
         Mem("too big", 1 << 50)
+
+
+def test_particles_resize(fs):
+    _announce("particles resize")
+    p = fs.Particles(10, 1.0)
+    p.append(np.arange(15, dtype=np.float64).reshape(5, 3) / 20.0)
+    p.resize(3)
+    assert len(p) == 3
+    assert np.allclose(p.x, np.arange(9).reshape(3, 3) / 20.0)
+    p.resize(8)
+    assert len(p) == 8
+    with pytest.raises(MemoryError):
+        p.resize(11)
+    with pytest.raises(AttributeError):
+        fs.pm.write_packet_info("packets.h5")
