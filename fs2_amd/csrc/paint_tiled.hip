@@ -41,6 +41,8 @@ constexpr int kBlock = 256;
 constexpr int kTileBlock = 512;
 constexpr int kReadoutBlock = 1024;
 constexpr unsigned kNoKey = 0xffffffffu;
+// log2 of the (x, y) tile-column group edge of the workgroup order
+constexpr int kTileGroupLog = 3;
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -85,6 +87,10 @@ struct TileGeom {
   int64_t mstride;
   // log2 of ntz, nty and ntiles when all three are powers of two, else -1
   int lntz, lnty, lnt;
+  // workgroups visit the (x, y) tile columns in 2^lg x 2^lg groups, z
+  // fastest, so a tile's +x and +y halo neighbours run close in time on
+  // the same XCD and their lines are still in its L2 (0: plain x-y-z order)
+  int lg;
 };
 
 // Tile of workgroup b (box b / ntiles, tile b % ntiles, z fastest); shifts
@@ -97,11 +103,16 @@ __device__ __forceinline__ TileCoord tile_coord(const TileGeom& g, unsigned b) {
   unsigned box, tile, q;
   if (g.lnt >= 0) {
     box = b >> g.lnt;
-    tile = b & ((1u << g.lnt) - 1u);
-    c.Tz = (int)(tile & ((1u << g.lntz) - 1u));
-    q = tile >> g.lntz;
-    c.Ty = (int)(q & ((1u << g.lnty) - 1u));
-    c.Tx = (int)(q >> g.lnty);
+    const unsigned w = b & ((1u << g.lnt) - 1u);
+    c.Tz = (int)(w & ((1u << g.lntz) - 1u));
+    q = w >> g.lntz;
+    // q = ((Tx_hi * nty_hi + Ty_hi) << 2lg) | (Ty_lo << lg) | Tx_lo
+    const unsigned gm = (1u << g.lg) - 1u;
+    const unsigned lo = q & ((1u << (2 * g.lg)) - 1u), hi = q >> (2 * g.lg);
+    const int lnty_hi = g.lnty - g.lg;
+    c.Tx = (int)(((hi >> lnty_hi) << g.lg) | (lo & gm));
+    c.Ty = (int)(((hi & ((1u << lnty_hi) - 1u)) << g.lg) | (lo >> g.lg));
+    tile = ((unsigned)c.Tx * (unsigned)g.nty + (unsigned)c.Ty) * (unsigned)g.ntz + (unsigned)c.Tz;
   } else {
     box = b / (unsigned)g.ntiles;
     tile = b - box * (unsigned)g.ntiles;
@@ -819,8 +830,10 @@ TileGeom make_geom(int n, float boxsize, const TileOpts& o, int64_t np = 0) {
     g.lntz = __builtin_ctz((unsigned)g.ntz);
     g.lnty = __builtin_ctz((unsigned)g.nty);
     g.lnt = __builtin_ctz((unsigned)g.ntiles);
+    g.lg = std::min({kTileGroupLog, g.lnty, __builtin_ctz((unsigned)g.ntx)});
   } else {
     g.lntz = g.lnty = g.lnt = -1;
+    g.lg = 0;
   }
   g.bins_box = (unsigned)g.ntiles * 8u;
   g.nbox = o.nbox < 1 ? 1 : o.nbox;
