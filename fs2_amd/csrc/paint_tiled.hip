@@ -10,8 +10,8 @@
 //               particle, key = tile * 8 + spill class; the class bits say
 //               whether the lower cell is on the last x / y / z layer of its
 //               tile, i.e. whether the +1 corner lands in the next tile.
-//               Bin counts use wavefront-aggregated atomics (one atomic per
-//               distinct key per wavefront).
+//               Bin counts are aggregated per workgroup in LDS (one
+//               atomic per distinct key and workgroup).
 //   K2 scan   : exclusive prefix sum of the counts -> bin offsets.
 //   K3 scatter: positions copied into bin order (wavefront-aggregated
 //               cursor atomics).
@@ -179,14 +179,6 @@ __device__ __forceinline__ unsigned wave_aggregated_inc(unsigned* ctr, unsigned 
   return base + lanes_below(same);
 }
 
-// Counting only: one atomic per distinct key and wavefront, result unused.
-__device__ __forceinline__ void wave_aggregated_count(unsigned* ctr, unsigned key, bool valid) {
-  int leader;
-  uint64_t same;
-  group_by_key(key, valid, &leader, &same);
-  if (valid && __lane_id() == leader) atomicAdd(ctr + key, (unsigned)__popcll(same));
-}
-
 __global__ void k_fill_u32(unsigned* __restrict__ a, int64_t n, unsigned v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -204,28 +196,92 @@ __device__ __forceinline__ unsigned bin_key(const TileGeom& g, float px, float p
   return key;
 }
 
+// K1 aggregates its counts per workgroup: kBinPer particles per thread
+// (kBinBlock x kBinPer consecutive particles per workgroup).  Within a
+// wavefront the lanes form runs of equal keys (consecutive particles are
+// neighbouring lattice sites); the first lane of a run finds the key's slot
+// in an LDS hash table (linear probing, claimed by compare-and-swap) and
+// adds the run's length there, and the workgroup then issues one global
+// atomic per distinct key.  The table has twice as many slots as the
+// workgroup has particles, so a probe always ends.  (The global atomics,
+// not the arithmetic, bound this kernel; the scatter, which needs a slot
+// per particle, keeps the per-wavefront grouping of wave_aggregated_inc.)
+constexpr int kBinBlock = 256;
+constexpr int kBinPer = 4;
+constexpr int kBinLogSlots = 11;
+constexpr int kBinSlots = 1 << kBinLogSlots;
+static_assert(kBinSlots >= 2 * kBinBlock * kBinPer, "bin hash table at most half full");
+inline unsigned bin_blocks(int64_t n) {
+  return (unsigned)((n + kBinBlock * kBinPer - 1) / (kBinBlock * kBinPer));
+}
+
+// On return *start is the lowest lane of this lane's run of equal keys and
+// *len the run's length (meaningful on the run's first lane).
+__device__ __forceinline__ void key_runs(unsigned key, int* start, int* len) {
+  const int lane = __lane_id();
+  const unsigned prev = (unsigned)__shfl_up((int)key, 1);
+  const uint64_t starts = __ballot(lane == 0 || prev != key);
+  // highest run start at or below this lane
+  const uint64_t below = starts & (~0ull >> (63 - lane));
+  *start = 63 - __builtin_clzll(below);
+  // next run start above this lane, or 64
+  const uint64_t above = lane == 63 ? 0ull : starts & (~0ull << (lane + 1));
+  *len = (above ? __builtin_ctzll(above) : 64) - lane;
+}
+
+__device__ __forceinline__ int bin_table_slot(unsigned* tkey, unsigned key) {
+  unsigned h = (key * 2654435761u) >> (32 - kBinLogSlots);
+  for (;;) {
+    const unsigned k = __atomic_load_n(&tkey[h], __ATOMIC_RELAXED);
+    if (k == key) return (int)h;
+    if (k == kNoKey) {
+      const unsigned old = atomicCAS(&tkey[h], kNoKey, key);
+      if (old == kNoKey || old == key) return (int)h;
+    }
+    h = (h + 1u) & (kBinSlots - 1u);
+  }
+}
+
 // K1: wrap (written back only where it changes x) and bin counts.  The
 // scatter recomputes the same key from the written positions, so no key
 // array is stored.
-__global__ void k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L, int do_wrap,
-                      TileGeom g, unsigned* __restrict__ counts) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < np;
-  unsigned key = kNoKey;
-  if (valid) {
-    float p[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      p[a] = x[i + a * stride];
-      if (do_wrap) {
-        const float w = wrap1(p[a], L);
-        if (w != p[a]) x[i + a * stride] = w;
-        p[a] = w;
-      }
-    }
-    key = bin_key(g, p[0], p[1], p[2], i);
+__global__ void __launch_bounds__(kBinBlock)
+k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L, int do_wrap, TileGeom g,
+      unsigned* __restrict__ counts) {
+  __shared__ unsigned tkey[kBinSlots], tcnt[kBinSlots];
+  for (int e = threadIdx.x; e < kBinSlots; e += blockDim.x) {
+    tkey[e] = kNoKey;
+    tcnt[e] = 0u;
   }
-  wave_aggregated_count(counts, key, valid);
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * (kBinBlock * kBinPer) + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    const int64_t i = i0 + k * kBinBlock;
+    unsigned key = kNoKey;
+    if (i < np) {
+      float p[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        p[a] = x[i + a * stride];
+        if (do_wrap) {
+          const float w = wrap1(p[a], L);
+          if (w != p[a]) x[i + a * stride] = w;
+          p[a] = w;
+        }
+      }
+      key = bin_key(g, p[0], p[1], p[2], i);
+    }
+    int start, len;
+    key_runs(key, &start, &len);
+    if (start == __lane_id() && key != kNoKey)
+      atomicAdd(&tcnt[bin_table_slot(tkey, key)], (unsigned)len);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kBinSlots; e += blockDim.x) {
+    const unsigned c = tcnt[e];
+    if (c) atomicAdd(counts + tkey[e], c);
+  }
 }
 
 // Exclusive scan of the bin counts in three launches: per-block scans of
@@ -883,7 +939,7 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
   launch("paint_clear_bins", (double)nbins * 4.0, k_fill_u32, dim3(blocks_for(nbins)),
          dim3(kBlock), 0, s.counts, nbins, 0u);
   if (p.np > 0) {
-    launch("paint_bin", (double)p.np * 12.0, k_bin, dim3(blocks_for(p.np)), dim3(kBlock), 0, p.x,
+    launch("paint_bin", (double)p.np * 12.0, k_bin, dim3(bin_blocks(p.np)), dim3(kBinBlock), 0, p.x,
            p.stride, p.np, boxsize, wrap ? 1 : 0, g, s.counts);
   }
   const int nb = (int)((nbins + kScanChunk - 1) / kScanChunk);
