@@ -30,10 +30,13 @@ def density(x, nc, boxsize, pm_factor):
     return cic.paint(x, nc, boxsize, pm_factor)
 
 
-def force_meshes(delta, boxsize):
-    """Three real force meshes f_a = irfft(i f2 k_a delta_k) (pm.cpp:460-508)."""
+def force_meshes(delta, boxsize, progress=None):
+    """Three real force meshes f_a = irfft(i f2 k_a delta_k) (pm.cpp:460-508).
+    progress(msg), if given, is called after each transform."""
     n = delta.shape[0]
     dk = np.fft.rfftn(delta)
+    if progress:
+        progress("r2c done")
     kxy, kz = int_kvectors_pm(n)
     KX = kxy[:, None, None]
     KY = kxy[None, :, None]
@@ -47,6 +50,8 @@ def force_meshes(delta, boxsize):
         fk = 1j * f2 * dk
         fk[0, 0, 0] = 0.0
         out.append(np.fft.irfftn(fk, s=(n, n, n), axes=(0, 1, 2)) * n**3)
+        if progress:
+            progress(f"c2r {len(out)} of 3 done")
     return out
 
 

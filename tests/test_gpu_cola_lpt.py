@@ -1,8 +1,11 @@
 """GPU parity: COLA kick/drift, 2LPT initial conditions, a full COLA run and
 the P(k) estimator against the FP64 oracle."""
+import os
+
 import numpy as np
 import pytest
 
+from oracle import cic as ocic
 from oracle import cola as ocola
 from oracle import cosmology as ocosmo
 from oracle import lpt as olpt
@@ -212,6 +215,54 @@ def test_cola_run_matches_oracle(fs, cosmo, eh):
     d -= L * np.round(d / L)
     cell = L / nc
     assert np.abs(d).max() / cell < 1e-3, np.abs(d).max() / cell
+
+
+def _final_forces_vs_fp64(fs, eh, n_p, nc, L):
+    """Forces after the 10-step COLA run against the FP64 oracle at the
+    clustered final positions; returns the relative RMS error."""
+    p = fs.lpt.init(n_p, L, 0.1, eh, 1, "cola")
+    fs.pm.init(nc, nc / n_p, L)
+    fs.pm.set_paint_variant("sorted")
+    for a_vel, a_pos in ocola.schedule(0.1, 1.0, 10):
+        fs.pm.force(p)
+        fs.cola.kick(p, a_vel)
+        fs.cola.drift(p, a_pos)
+    fs.pm.force(p)
+    x = p.x.astype(np.float64)   # wrapped into the box by the force call
+    f = p.force.astype(np.float64)
+    del p
+    # opm.force in its steps, each announced (the large case runs minutes)
+    delta = opm.density(x, nc, L, nc / n_p)
+    _announce("oracle density done")
+    meshes = opm.force_meshes(delta, L, progress=lambda m: _announce(f"oracle {m}"))
+    del delta
+    _announce("oracle force meshes done")
+    ref = ocic.readout(x, meshes, L)
+    _announce("oracle readout done")
+    return float(np.sqrt(np.mean((f - ref) ** 2) / np.mean(ref ** 2)))
+
+
+@pytest.mark.timeout(900)
+def test_config_b_forces_after_cola_run_match_fp64(fs, cosmo, eh):
+    """256^3 particles, 512^3 mesh, box 512 Mpc/h: after the 10-step COLA run
+    (a = 0.1 -> 1), the PM forces at the clustered final positions match the
+    FP64 oracle (paint, FP64 FFTs, Green, readout) to relative RMS <= 1e-5."""
+    _announce("config B forces vs FP64 oracle (256^3 p / 512^3 mesh)")
+    rel = _final_forces_vs_fp64(fs, eh, 256, 512, 512.0)
+    print(f"[case] config B force rel RMS = {rel:.3e}", flush=True)
+    assert rel <= 1e-5, rel
+
+
+@pytest.mark.timeout(1800)
+@pytest.mark.skipif(os.environ.get("FS2_LARGE_TESTS") != "1",
+                    reason="512^3 / 1024^3 FP64 oracle: minutes and ~60 GB of host memory; "
+                           "set FS2_LARGE_TESTS=1")
+def test_config_c_forces_after_cola_run_match_fp64(fs, cosmo, eh):
+    """The headline configuration: 512^3 particles, 1024^3 mesh, box 1024."""
+    _announce("config C forces vs FP64 oracle (512^3 p / 1024^3 mesh)")
+    rel = _final_forces_vs_fp64(fs, eh, 512, 1024, 1024.0)
+    print(f"[case] config C force rel RMS = {rel:.3e}", flush=True)
+    assert rel <= 1e-5, rel
 
 
 def test_power_spectrum_matches_oracle(fs, cosmo, eh):
