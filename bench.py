@@ -329,9 +329,11 @@ def main_leg(args, fs, ps):
         if launches == 0:
             continue
         avg = total_ms / launches
+        gbps = bytes_per / (avg * 1e-3) / 1e9
         kernels[name] = {"launches": launches, "avg_ms": round(avg, 4),
                          "total_ms": round(total_ms, 3),
-                         "GBps": round(bytes_per / (avg * 1e-3) / 1e9, 1)}
+                         "algorithmic_GB": round(bytes_per / 1e9, 3),
+                         "GBps": round(gbps, 1), "frac_hbm": round(gbps / PEAK_HBM_GBS, 3)}
     dom = max(prof, key=lambda r: r[2])
     dom_avg_s = dom[2] / dom[1] * 1e-3
     achieved = dom[3] / dom_avg_s / 1e9
@@ -414,6 +416,8 @@ def main():
         out["copy_peak"] = copy_probe(fs)
         out["roofline"]["frac_of_copy_peak"] = round(
             out["roofline"]["achieved"] / out["copy_peak"]["GBps"], 4)
+        for k in out["kernels"].values():
+            k["frac_copy_peak"] = round(k["GBps"] / out["copy_peak"]["GBps"], 3)
     if "adjoint" in legs:
         print("[bench] adjoint 128^3/256^3", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(fs, ps)
