@@ -143,15 +143,35 @@ template <> struct Plan<1024> { static constexpr int T = 64; using R = Radices<1
 // LDS layouts of one pencil (element j at s[L::at(j)]).  The padding
 // breaks the power-of-two strides of the Stockham writes (rows R apart in
 // the first stage) across LDS banks.
+//   at(j + m) = at(j) + span(m)  for m a multiple of kPeriod (any j >= 0);
+//   at(x + r) = at(x) + r * kUnit  for x a multiple of R, r < R, R | kPeriod;
+// the stages use these to address a thread's R elements from one base with
+// compile-time offsets (immediate LDS offsets) instead of R index
+// computations.
 template <int P>
 struct RowsLayout {   // [j][P] tiles of strided pencils, one pad row per 16
+  static constexpr int kPeriod = 16, kUnit = P;
   static constexpr int rows(int N) { return N + N / 16; }
+  static constexpr int span(int m) { return (m + m / 16) * P; }
   static __device__ __forceinline__ int at(int j) { return (j + (j >> 4)) * P; }
 };
 struct ContigLayout {   // contiguous pencil, one pad element per 8
+  static constexpr int kPeriod = 8, kUnit = 1;
   static constexpr int size(int N) { return N + N / 8 + 1; }
+  static constexpr int span(int m) { return m + m / 8; }
   static __device__ __forceinline__ int at(int j) { return j + (j >> 3); }
 };
+
+// s[at(j + r m)] for r < R, as one base plus constant offsets when the
+// layout allows it.
+template <class L, int M>
+__device__ __forceinline__ int lds_elem(int a0, int j, int r) {
+  if constexpr (M % L::kPeriod == 0) {
+    return a0 + r * L::span(M);
+  } else {
+    return L::at(j + r * M);
+  }
+}
 
 // One Stockham stage over one pencil held in LDS at s[L::at(j)].
 template <int N, int T, class L, int Ns, int R>
@@ -162,8 +182,9 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const int j = t + b * T;
+    const int a0 = L::at(j);
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = s[L::at(j + r * (N / R))];
+    for (int r = 0; r < R; ++r) v[b][r] = s[lds_elem<L, N / R>(a0, j, r)];
   }
   __syncthreads();
 #pragma unroll
@@ -176,8 +197,9 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
     }
     DFT<R>::run(v[b]);
     const int base = (j - k) * R + k;
+    const int a0 = L::at(base);
 #pragma unroll
-    for (int r = 0; r < R; ++r) s[L::at(base + r * Ns)] = v[b][r];
+    for (int r = 0; r < R; ++r) s[lds_elem<L, Ns>(a0, base, r)] = v[b][r];
   }
   __syncthreads();
 }
@@ -214,8 +236,9 @@ __device__ __forceinline__ void stage_last(float2* x, const float2* s, int t,
   for (int b = 0; b < B; ++b) {
     const int j = t + b * T;   // j < Ns, so the output base is j itself
     float2 v[R];
+    const int a0 = L::at(j);
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = s[L::at(j + r * (N / R))];
+    for (int r = 0; r < R; ++r) v[r] = s[lds_elem<L, N / R>(a0, j, r)];
 #pragma unroll
     for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * j * (N / (Ns * R))]);
     DFT<R>::run(v);
