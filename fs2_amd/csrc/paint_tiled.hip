@@ -255,22 +255,29 @@ k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L, int do_wrap, T
   }
   __syncthreads();
   const int64_t i0 = (int64_t)blockIdx.x * (kBinBlock * kBinPer) + threadIdx.x;
+  // every position load first: the wrap stores below could alias later
+  // loads as far as the compiler knows, which would serialise them
+  float p[kBinPer][3];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    const int64_t i = i0 + k * kBinBlock;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[k][a] = i < np ? x[i + a * stride] : 0.f;
+  }
 #pragma unroll
   for (int k = 0; k < kBinPer; ++k) {
     const int64_t i = i0 + k * kBinBlock;
     unsigned key = kNoKey;
     if (i < np) {
-      float p[3];
+      if (do_wrap) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        p[a] = x[i + a * stride];
-        if (do_wrap) {
-          const float w = wrap1(p[a], L);
-          if (w != p[a]) x[i + a * stride] = w;
-          p[a] = w;
+        for (int a = 0; a < 3; ++a) {
+          const float w = wrap1(p[k][a], L);
+          if (w != p[k][a]) x[i + a * stride] = w;
+          p[k][a] = w;
         }
       }
-      key = bin_key(g, p[0], p[1], p[2], i);
+      key = bin_key(g, p[k][0], p[k][1], p[k][2], i);
     }
     int start, len;
     key_runs(key, &start, &len);
