@@ -611,6 +611,56 @@ k_fft_z_r2c(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
   }
 }
 
+// The same r2c with the M-point and split twiddles staged in LDS once per
+// block of P rows (for large M, where every one-row block would otherwise
+// fetch ~20 twiddles per thread from L2).
+template <int M, int P>
+__global__ void __launch_bounds__(Plan<M>::T * P)
+k_fft_z_r2c_lt(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ twM,
+               const float2* __restrict__ twN) {
+  constexpr int T = Plan<M>::T;
+  using L = ContigLayout;
+  constexpr int LS = L::size(M);
+  __shared__ float2 s[P * LS];
+  __shared__ float2 stM[M];
+  __shared__ float2 stN[M];
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    stM[i] = twM[i];
+    stN[i] = twN[i];
+  }
+  const int tid = threadIdx.x;
+  const int t = tid % T, p = tid / T;
+  const int64_t row = (int64_t)blockIdx.x * P + p;
+  const bool valid = row < nrows;
+  float2* r = data + row * cpitch;
+  float2* sp = s + p * LS;
+  float2 x[M / T];
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int m = t + i * T;
+    x[i] = valid ? r[m] : make_float2(0.f, 0.f);
+  }
+  fft_regs<M, L>(x, sp, t, stM);   // starts with a barrier: the twiddles are staged
+  __syncthreads();   // every thread is done reading the last stage's input
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = x[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int k = t + i * T;
+    const float2 A = sp[L::at(k)];
+    const float2 Bc = cconj(sp[L::at(k == 0 ? 0 : M - k)]);
+    const float2 E = cscale(cadd(A, Bc), 0.5f);
+    const float2 O = cscale(csub(A, Bc), 0.5f);
+    const float2 X = cadd(E, mul_negi(cmul(stN[k], O)));
+    if (valid) r[k] = X;
+  }
+  if (t == 0 && valid) {
+    const float2 A = sp[L::at(0)];
+    r[M] = make_float2(A.x - A.y, 0.f);
+  }
+}
+
 // z pass, c2r: the inverse of the split above, with the imaginary parts of
 // X[0] and X[M] dropped:  Z[k] = E' + i O',  E' = X[k] + conj X[M-k],
 // O' = (X[k] - conj X[M-k]) W^{-k};  z = IDFT_M(Z) = x_even + i x_odd.
@@ -735,6 +785,8 @@ const float2* twiddles(int N) {
 template <int N> constexpr int pencil_cols() {
   return ((N + N / 16) * 16 * 8 <= 81920 && Plan<N>::T * 16 <= 1024) ? 16 : 8;
 }
+// rows per block of the LDS-twiddle r2c z pass (M >= 256)
+constexpr int kZR2CRows = 2;
 template <int M> constexpr int zrows() {   // one-wave blocks from M = 512 on (measured)
   return Plan<M>::T >= 256 ? 1 : (M >= 512 ? 64 : 256) / Plan<M>::T;
 }
@@ -782,9 +834,16 @@ struct Impl {
     const int64_t nrows = (int64_t)n * n * nb;
     const double bytes = ((double)n * n * n * 4.0 + grid_bytes()) * nb;
     // (a persistent r2c, like the c2r below, measured slower at 1024^3)
-    launch("fft_z_r2c", bytes, k_fft_z_r2c<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
-           dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
-           twiddles(N));
+    if constexpr (M >= 256) {
+      constexpr int PL = kZR2CRows;
+      launch("fft_z_r2c", bytes, k_fft_z_r2c_lt<M, PL>, dim3((unsigned)((nrows + PL - 1) / PL)),
+             dim3(Plan<M>::T * PL), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
+             twiddles(N));
+    } else {
+      launch("fft_z_r2c", bytes, k_fft_z_r2c<M, PZ>, dim3((unsigned)((nrows + PZ - 1) / PZ)),
+             dim3(Plan<M>::T * PZ), 0, (float2*)grid, nrows, grid_cpitch(n), twiddles(M),
+             twiddles(N));
+    }
   }
 
   static void z_c2r(float* grid, int nb) {
