@@ -159,3 +159,33 @@ def test_no_gpu_fails_loudly(fs):
     """No CPU fallback: device entry points raise without a GPU."""
     with pytest.raises(RuntimeError, match="no HIP device"):
         fs.Particles(10, 1.0)
+
+
+def test_error_paths_without_a_device():
+    """Argument and state errors come back as status codes with a message,
+    before any device work (no GPU is touched here)."""
+    import torch  # noqa: F401  (one HIP runtime, as the package loads it)
+    lib = ctypes.CDLL(LIB)
+    lib.fs2_last_error.restype = ctypes.c_char_p
+    FS2_ERR_VALUE, FS2_ERR_RUNTIME = 5, 1
+    u64 = ctypes.c_uint64
+    # handle checks
+    for name in ("fs2_cola_kick", "fs2_cola_drift"):
+        f = getattr(lib, name)
+        f.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        assert f(None, 0.5) == FS2_ERR_VALUE
+        assert b"null particles handle" in lib.fs2_last_error()
+    lib.fs2_particles_resize.argtypes = [ctypes.c_void_p, u64]
+    assert lib.fs2_particles_resize(None, 3) == FS2_ERR_VALUE
+    lib.fs2_mem_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    assert lib.fs2_mem_info(None, None, None) == FS2_ERR_VALUE
+    lib.fs2_copy_bandwidth.argtypes = [u64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    assert lib.fs2_copy_bandwidth(1 << 20, 1024, 1, None) == FS2_ERR_VALUE
+    # FoF state queries before any run: no groups, no tree
+    n = u64(123)
+    lib.fs2_fof_grp.argtypes = [ctypes.c_void_p, ctypes.POINTER(u64)]
+    assert lib.fs2_fof_grp(None, ctypes.byref(n)) == 0 and n.value == 0
+    h = u64()
+    lib.fs2_kdtree_height.argtypes = [ctypes.POINTER(u64)]
+    assert lib.fs2_kdtree_height(ctypes.byref(h)) == FS2_ERR_RUNTIME
+    assert b"no kd-tree" in lib.fs2_last_error()
