@@ -337,9 +337,9 @@ def main_leg(args, fs, ps):
     dom = max(prof, key=lambda r: r[2])
     dom_avg_s = dom[2] / dom[1] * 1e-3
     achieved = dom[3] / dom_avg_s / 1e9
-    ng = args.nc**3
-    nk = args.nc**2 * (args.nc // 2 + 1)
-    step_bytes = 156.0 * npart + 32.0 * ng + 32.0 * nk
+    # algorithmic bytes of one step: every timed launch's declared bytes
+    # (164 Np + 32 Ng + 112 Nk, Nk = n^2 (n/2+1), for the COLA step)
+    step_bytes = sum(r[3] * r[1] for r in prof) / args.steps
     headline = (args.np, args.nc, args.box) == (512, 1024, 1024.0)
     return {
         "value": npart * args.steps / T,
