@@ -89,6 +89,8 @@ void set_profiling(bool on);
 template <typename K, typename... Args>
 inline void launch(const char* name, double bytes, K kernel, dim3 grid, dim3 block,
                    size_t shmem, Args... args) {
+  // a grid sized from an empty input (no particles) has no work to do
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
   int tok = prof_begin(name, bytes);
   hipLaunchKernelGGL(kernel, grid, block, shmem, stream(), args...);
   hipError_t e = hipGetLastError();

@@ -210,6 +210,35 @@ def test_compute_force_requires_density(fs):
         fs.pm.compute_force(p)
 
 
+@pytest.mark.parametrize("variant", ["sorted", "atomic", "global"])
+def test_empty_particle_set(fs, variant):
+    """No particles (an empty domain in the reference's loops): the density
+    is -1 in every cell, the force call and a COLA kick + drift run and
+    leave an empty set, and particles appended afterwards paint as usual."""
+    _announce(f"empty particle set, paint {variant}")
+    L, nc = 64.0, 16
+    fs.cosmology.init(0.31)
+    fs.pm.init(nc, 2.0, L)
+    fs.pm.set_paint_variant(variant)
+    try:
+        p = fs.Particles(8, L)
+        fs.pm.send_positions(p)
+        assert np.all(fs.pm.compute_density(p).asarray() == -1.0)
+        p.set_times(0.5, 0.5, 0.0)
+        fs.pm.force(p)
+        assert p.force.shape == (0, 3) and p.x.shape == (0, 3)
+        fs.cola.kick(p, 0.55)
+        fs.cola.drift(p, 0.6)
+        assert len(p) == 0 and p.v.shape == (0, 3)
+        assert p.times[:2] == (0.6, 0.55)
+        p.append(np.array([[1.0, 2.0, 3.0]]))
+        fs.pm.send_positions(p)
+        d = fs.pm.compute_density(p).asarray().astype(np.float64)
+        assert abs(d.sum() - (8.0 - nc**3)) < 1e-3     # one particle of mass pm_factor^3
+    finally:
+        fs.pm.set_paint_variant("sorted")
+
+
 def _clustered(npart, L, rng):
     """Half uniform, half in a few tight clumps, plus particles on tile and
     box faces (lower cells on the last layer of a tile spill into the next)."""
