@@ -189,3 +189,14 @@ def test_error_paths_without_a_device():
     lib.fs2_kdtree_height.argtypes = [ctypes.POINTER(u64)]
     assert lib.fs2_kdtree_height(ctypes.byref(h)) == FS2_ERR_RUNTIME
     assert b"no kd-tree" in lib.fs2_last_error()
+    # PM parameters (py/pm.cpp:245-255), checked before any allocation
+    pm = ctypes.c_void_p()
+    lib.fs2_pm_create.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.POINTER(ctypes.c_void_p)]
+    assert lib.fs2_pm_create(1, 1.0, 64.0, ctypes.byref(pm)) == FS2_ERR_RUNTIME
+    assert b"must be larger than 1" in lib.fs2_last_error()
+    assert lib.fs2_pm_create(16, 1.0, 0.0, ctypes.byref(pm)) == FS2_ERR_RUNTIME
+    assert b"must be positive" in lib.fs2_last_error()
+    assert lib.fs2_pm_create(20, 1.0, 64.0, ctypes.byref(pm)) == FS2_ERR_VALUE
+    assert b"not a supported FFT size" in lib.fs2_last_error()
+    assert pm.value is None
