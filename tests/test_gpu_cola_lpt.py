@@ -217,6 +217,59 @@ def test_cola_run_matches_oracle(fs, cosmo, eh):
     assert np.abs(d).max() / cell < 1e-3, np.abs(d).max() / cell
 
 
+def test_config_a_run_forces_and_power_match_fp64(fs, cosmo, eh):
+    """The reference's CPU case: 64^3 particles, 128^3 mesh, box 64 Mpc/h,
+    2LPT at a = 0.1, 10 COLA steps to a = 1, beside an FP64 oracle
+    trajectory from the same 2LPT state.  Positions: RMS below 1e-4 cells,
+    99.99% within 1e-3 cells, none beyond 1e-2 (the FP32 floor of dense
+    regions, as at 256^3); forces at the final positions within 1e-5
+    relative RMS of FP64; final P(k) within 0.1%."""
+    _announce("config A: 64^3 p / 128^3 mesh, 10 steps vs FP64 oracle")
+    n_p, nc, L = 64, 128, 64.0
+    p = fs.lpt.init(n_p, L, 0.1, eh, 21, "cola")
+    x, v = p.x.astype(np.float64), p.v.astype(np.float64)
+    dx1, dx2 = p.dx1.astype(np.float64), p.dx2.astype(np.float64)
+    fs.pm.init(nc, nc / n_p, L)
+    fs.pm.set_paint_variant("sorted")
+    a_x = a_v = 0.1
+    for a_vel, a_pos in ocola.schedule(0.1, 1.0, 10):
+        fs.pm.force(p)
+        fs.cola.kick(p, a_vel)
+        fs.cola.drift(p, a_pos)
+        xw = np.mod(x, L)
+        f = opm.force(xw, nc, L, nc / n_p)
+        v = ocola.kick(v, f, dx1, dx2, cosmo, a_v, a_x, a_vel)
+        a_v = a_vel
+        x = ocola.drift(xw, v, dx1, dx2, cosmo, a_x, a_v, a_pos)
+        a_x = a_pos
+    d = p.x.astype(np.float64) - x
+    d -= L * np.round(d / L)
+    cell = L / nc
+    e = np.abs(d).max(axis=1) / cell
+    rms = float(np.sqrt(np.mean(d ** 2)) / cell)
+    print(f"[case] config A position error (cells): rms {rms:.2e} "
+          f"p99.99 {np.quantile(e, 0.9999):.2e} max {e.max():.2e}", flush=True)
+    assert rms < 1e-4 and np.quantile(e, 0.9999) < 1e-3 and e.max() < 1e-2
+    # forces at the GPU's final positions
+    p.set_times(1.0, 1.0, 0.0)
+    fs.pm.force(p)
+    xg = p.x.astype(np.float64)
+    ref = opm.force(xg, nc, L, nc / n_p)
+    rel = float(np.sqrt(np.mean((p.force - ref) ** 2) / np.mean(ref ** 2)))
+    print(f"[case] config A final force rel RMS = {rel:.3e}", flush=True)
+    assert rel <= 1e-5, rel
+    # P(k) of the final density against the oracle's trajectory
+    fs.pm.send_positions(p)
+    fs.pm.compute_density(p)
+    k, P, nm = fs.pm.power_spectrum(nc=nc)
+    kr, Pr, nmr = opk.power_spectrum(opm.density(np.mod(x, L), nc, L, nc / n_p), L)
+    ok = nmr > 0
+    assert np.array_equal(nm, nmr)
+    perr = float(np.abs(P[ok] / Pr[ok] - 1).max())
+    print(f"[case] config A max P(k) relative error = {perr:.3e}", flush=True)
+    assert perr < 1e-3, perr
+
+
 def _final_forces_vs_fp64(fs, eh, n_p, nc, L):
     """Forces after the 10-step COLA run against the FP64 oracle at the
     clustered final positions; returns the relative RMS error."""
