@@ -384,6 +384,7 @@ __global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t n
     key = bin_key(g, p[0], p[1], p[2], i);
   }
   const unsigned slot = wave_aggregated_inc(cursor, key, valid);
+  // the index fits in 32 bits: 2^32 particles (68 bytes each) exceed the HBM
   if (valid) rec[slot] = make_float4(p[0], p[1], p[2], __uint_as_float((unsigned)i));
 }
 
