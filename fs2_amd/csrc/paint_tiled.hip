@@ -498,27 +498,13 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
   extern __shared__ unsigned char smem[];
   ACC* acc = (ACC*)smem;
   const int tcells = g.tx * g.ty * g.tz;
+  lds_zero(smem, tcells * (int)sizeof(ACC));
 
   // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
   const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
   offsets += (int64_t)tc.box * g.bins_box;
   const int x0 = tc.Tx * g.tx, y0 = tc.Ty * g.ty, z0 = tc.Tz * g.tz;
   float* gt = grid + (int64_t)tc.box * g.mstride + ((int64_t)x0 * g.n + y0) * g.ncz + z0;
-
-  // The tile's own records (all its classes: one contiguous range, the
-  // first of the source ranges) are loaded for the first round before the
-  // LDS clear and the source-range setup, so their latency overlaps both.
-  const unsigned own_b = offsets[(unsigned)tc.tile * 8u];
-  const unsigned own = offsets[(unsigned)tc.tile * 8u + 8u] - own_b;
-  float4 q[2];
-  bool pre[2];
-#pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const unsigned v = threadIdx.x + u * blockDim.x;
-    pre[u] = v < own;
-    if (pre[u]) q[u] = rec[own_b + v];
-  }
-  lds_zero(smem, tcells * (int)sizeof(ACC));
 
   __shared__ unsigned rb[64], rpre[65];
   __shared__ int nranges;
@@ -562,16 +548,16 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
   };
   // two records per thread per round, both loads issued before either is
   // deposited (a tile holds ~1.3 records per thread at 1 particle per 8
-  // cells, so most threads need one round); record v < own of the first
-  // round is already in q
+  // cells, so most threads need one round)
   int r = 0;
   for (unsigned v0 = threadIdx.x; v0 < total; v0 += 2 * blockDim.x) {
+    float4 q[2];
     bool ok[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const unsigned v = v0 + u * blockDim.x;
       ok[u] = v < total;
-      if (ok[u] && !(v0 == threadIdx.x && pre[u])) {
+      if (ok[u]) {
         while (v >= rpre[r + 1]) ++r;
         q[u] = rec[rb[r] + (v - rpre[r])];
       }
