@@ -13,8 +13,8 @@
 //               Bin counts are aggregated per workgroup in LDS (one
 //               atomic per distinct key and workgroup).
 //   K2 scan   : exclusive prefix sum of the counts -> bin offsets.
-//   K3 scatter: positions copied into bin order (wavefront-aggregated
-//               cursor atomics).
+//   K3 scatter: positions copied into bin order (slots reserved per
+//               workgroup: one cursor atomic per distinct key).
 //   K4 tile   : per tile, LDS accumulation of every stencil corner that
 //               lands in the tile, from the tile's own particles and from the
 //               spilling classes of its lower neighbours; then one coalesced
@@ -135,50 +135,6 @@ __device__ __forceinline__ unsigned key_of(const TileGeom& g, int ix, int iy, in
   return tile * 8u + cls;
 }
 
-// Lanes of a wavefront grouped by equal key: on return *leader is the
-// lowest lane holding this lane's key and *same the mask of those lanes.
-// One round per distinct key: the key of the lowest ungrouped lane is read
-// into a scalar register and compared by every lane; no memory traffic.
-__device__ __forceinline__ void group_by_key(unsigned key, bool valid, int* leader,
-                                             uint64_t* same) {
-  uint64_t active = __ballot(valid);
-  int ld = __lane_id();
-  uint64_t sm = 0;
-  while (active) {
-    const int l = __builtin_ctzll(active);
-    const unsigned lk = (unsigned)__builtin_amdgcn_readlane((int)key, l);
-    const bool mine = valid && key == lk;
-    const uint64_t m = __ballot(mine);
-    if (mine) {
-      ld = l;
-      sm = m;
-    }
-    active &= ~m;
-  }
-  *leader = ld;
-  *same = sm;
-}
-
-// Number of lanes below this one in mask m.
-__device__ __forceinline__ unsigned lanes_below(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
-
-// Wavefront-aggregated atomicAdd of 1 per lane to ctr[key]; returns the
-// slot of this lane (old value + rank among lanes with the same key).  The
-// group leaders issue their atomics together, so a wavefront waits for one
-// atomic round trip however many distinct keys it holds.
-__device__ __forceinline__ unsigned wave_aggregated_inc(unsigned* ctr, unsigned key, bool valid) {
-  int leader;
-  uint64_t same;
-  group_by_key(key, valid, &leader, &same);
-  const int lane = __lane_id();
-  unsigned base = 0;
-  if (valid && lane == leader) base = atomicAdd(ctr + key, (unsigned)__popcll(same));
-  base = __shfl(base, leader);
-  return base + lanes_below(same);
-}
-
 __global__ void k_fill_u32(unsigned* __restrict__ a, int64_t n, unsigned v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -204,8 +160,8 @@ __device__ __forceinline__ unsigned bin_key(const TileGeom& g, float px, float p
 // adds the run's length there, and the workgroup then issues one global
 // atomic per distinct key.  The table has twice as many slots as the
 // workgroup has particles, so a probe always ends.  (The global atomics,
-// not the arithmetic, bound this kernel; the scatter, which needs a slot
-// per particle, keeps the per-wavefront grouping of wave_aggregated_inc.)
+// not the arithmetic, bound this kernel; the scatter reserves its slots
+// over the same table.)
 // Clustering breaks up the runs of consecutive particles: at a = 1 a
 // workgroup of 1024 particles holds 150 distinct keys against 47 at
 // a = 0.1.  Larger workgroups share keys between more of their rows, so
@@ -377,21 +333,64 @@ __global__ void k_scan_add(const unsigned* partial, const unsigned* __restrict__
 
 // One 16-byte record per particle in bin order: position and the particle's
 // index (bits in .w), which the tiled readout uses to write forces back in
-// id order.
-__global__ void k_scatter(const float* __restrict__ x, int64_t stride, int64_t np, TileGeom g,
-                          unsigned* __restrict__ cursor, float4* __restrict__ rec) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < np;
-  float p[3] = {0.f, 0.f, 0.f};
-  unsigned key = kNoKey;
-  if (valid) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) p[a] = x[i + a * stride];
-    key = bin_key(g, p[0], p[1], p[2], i);
+// id order.  Slots are reserved per workgroup, over the particles and the
+// table of K1: a run of equal keys in a wavefront takes its ranks within the
+// workgroup from one LDS atomic, the workgroup reserves one contiguous range
+// per distinct key with one global atomic, and each particle stores its
+// record at the range start plus its rank.
+__global__ void __launch_bounds__(kBinBlock)
+k_scatter(const float* __restrict__ x, int64_t stride, int64_t np, TileGeom g,
+          unsigned* __restrict__ cursor, float4* __restrict__ rec) {
+  __shared__ unsigned tkey[kBinSlots], tcnt[kBinSlots];
+  for (int e = threadIdx.x; e < kBinSlots; e += blockDim.x) {
+    tkey[e] = kNoKey;
+    tcnt[e] = 0u;
   }
-  const unsigned slot = wave_aggregated_inc(cursor, key, valid);
-  // the index fits in 32 bits: 2^32 particles (68 bytes each) exceed the HBM
-  if (valid) rec[slot] = make_float4(p[0], p[1], p[2], __uint_as_float((unsigned)i));
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * (kBinBlock * kBinPer) + threadIdx.x;
+  float p[kBinPer][3];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    const int64_t i = i0 + k * kBinBlock;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[k][a] = i < np ? x[i + a * stride] : 0.f;
+  }
+  const int lane = __lane_id();
+  int slot[kBinPer];
+  unsigned rank[kBinPer];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    const int64_t i = i0 + k * kBinBlock;
+    const unsigned key = i < np ? bin_key(g, p[k][0], p[k][1], p[k][2], i) : kNoKey;
+    int start, len;
+    key_runs(key, &start, &len);
+    int sl = -1;
+    unsigned r0 = 0u;
+    if (start == lane && key != kNoKey) {
+      sl = bin_table_slot(tkey, key);
+      r0 = atomicAdd(&tcnt[sl], (unsigned)len);
+    }
+    sl = __shfl(sl, start);
+    r0 = (unsigned)__shfl((int)r0, start);
+    slot[k] = key != kNoKey ? sl : -1;
+    rank[k] = r0 + (unsigned)(lane - start);
+  }
+  __syncthreads();
+  // one global atomic per distinct key; the count becomes the range start
+  for (int e = threadIdx.x; e < kBinSlots; e += blockDim.x) {
+    const unsigned c = tcnt[e];
+    if (c) tcnt[e] = atomicAdd(cursor + tkey[e], c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    if (slot[k] >= 0) {
+      const int64_t i = i0 + k * kBinBlock;
+      // the index fits in 32 bits: 2^32 particles (68 bytes each) exceed the HBM
+      rec[tcnt[slot[k]] + rank[k]] =
+          make_float4(p[k][0], p[k][1], p[k][2], __uint_as_float((unsigned)i));
+    }
+  }
 }
 
 // Source tiles S = T - d, d in {0,1}^3 (periodic).  A particle of S
@@ -983,8 +982,8 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
          dim3(kBlock), 0, (const unsigned*)s.cursor, (const unsigned*)s.block_sums, nbins,
          s.offsets, s.cursor);
   if (p.np > 0) {
-    launch("paint_scatter", (double)p.np * 28.0, k_scatter, dim3(blocks_for(p.np)), dim3(kBlock),
-           0, (const float*)p.x, p.stride, p.np, g, s.cursor, s.rec);
+    launch("paint_scatter", (double)p.np * 28.0, k_scatter, dim3(bin_blocks(p.np)),
+           dim3(kBinBlock), 0, (const float*)p.x, p.stride, p.np, g, s.cursor, s.rec);
   }
   const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0 * g.nbox;
   // fixed-point unit 2^-k with fac * 2^k <= 2^30, so every term fits an int32
