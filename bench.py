@@ -385,6 +385,8 @@ def main():
               f"(the multi-GPU slab decomposition is design-only).", file=sys.stderr)
         sys.exit(2)
 
+    from fs2_amd import build as _build
+    _build.ensure_built()
     import fs2_amd.fs as fs
     fs.cosmology.init(0.31)
     ps = fs.PowerSpectrum.eisenstein_hu(0.31, 0.049, 0.67, 0.96, 0.81)

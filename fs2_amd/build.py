@@ -101,6 +101,13 @@ def build_all(jobs=8):
     build_pyext()
 
 
+def ensure_built(jobs=8):
+    """Builds only when libfs2hip.so or the _fs module is missing (a fresh
+    checkout); an existing build is loaded as it is."""
+    if not (os.path.exists(LIB) and os.path.exists(PYEXT)):
+        build_all(jobs)
+
+
 if __name__ == '__main__':
     build_all()
     print('built', LIB, PYEXT)
