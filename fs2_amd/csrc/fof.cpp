@@ -34,6 +34,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <exception>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -58,13 +60,32 @@ int host_threads() {
   return std::max(1, std::min(n, 64));
 }
 
-// Runs f(t) for t < nt on nt threads (the caller's thread takes t = 0).
+// Runs f(t) for t < nt on nt threads (the caller's thread takes t = 0, and
+// any t whose thread cannot be started).  Every thread is joined before an
+// exception of any f(t) is rethrown on the caller's thread.
 template <typename F>
 void parallel(int nt, F f) {
+  std::vector<std::exception_ptr> err((size_t)std::max(nt, 1));
+  auto run = [&](int t) {
+    try {
+      f(t);
+    } catch (...) {
+      err[(size_t)t] = std::current_exception();
+    }
+  };
   std::vector<std::thread> th;
-  for (int t = 1; t < nt; ++t) th.emplace_back(f, t);
-  f(0);
+  th.reserve((size_t)std::max(nt - 1, 0));
+  for (int t = 1; t < nt; ++t) {
+    try {
+      th.emplace_back(run, t);
+    } catch (const std::system_error&) {
+      run(t);
+    }
+  }
+  run(0);
   for (auto& x : th) x.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 struct Builder {
@@ -120,9 +141,15 @@ struct Builder {
     tree[il].k = k;
     tree[ir].k = k;
     float lo1[3], hi1[3];
+    std::thread th;
     if (spawn > 0) {
-      std::thread th([&] { build(il, ib, mid, lo, hi, spawn - 1); });
-      build(ir, mid, ie, lo1, hi1, spawn - 1);
+      try {
+        th = std::thread([&] { build(il, ib, mid, lo, hi, spawn - 1); });
+      } catch (const std::system_error&) {   // no thread: build both here
+      }
+    }
+    if (th.joinable()) {
+      build(ir, mid, ie, lo1, hi1, spawn - 1);   // build never throws
       th.join();
     } else {
       build(il, ib, mid, lo, hi, 0);
