@@ -75,15 +75,21 @@ typedef struct fs2_diff fs2_diff;
 typedef struct fs2_batch fs2_batch;
 
 /* ---- library ---------------------------------------------------------- */
+/* Message of the last failed call on this thread (no reference counterpart:
+ * the reference raises a Python exception, py/error.h:4-14). */
 const char* fs2_last_error(void);
+/* Library version (no reference counterpart). */
 int fs2_version(void);
 /* py/py_msg.cpp set_loglevel: 0 debug .. 4 fatal (py/msg.h:4). */
 int fs2_set_loglevel(int level);
-/* Selects device 0 and creates the library stream.  FS2_ERR_DEVICE when no
- * GPU is present.  Called implicitly by every entry point that needs it. */
+/* Selects device 0 and creates the library stream (no reference
+ * counterpart).  FS2_ERR_DEVICE when no GPU is present.  Called implicitly by
+ * every entry point that needs it. */
 int fs2_device_init(void);
+/* Waits for the library stream (no reference counterpart). */
 int fs2_synchronize(void);
-/* The library stream as a hipStream_t cast to void*. */
+/* The library stream as a hipStream_t cast to void* (no reference
+ * counterpart). */
 int fs2_get_stream(void** stream);
 
 /* ---- cosmology (FP64, host) ------------------------------------------- */
@@ -115,6 +121,7 @@ int fs2_power_alloc(const char* filename, fs2_power** out);
  * to sigma8 (top hat R = 8 Mpc/h), k in h/Mpc. */
 int fs2_power_alloc_eh98(double omega_m, double omega_b, double h, double ns,
                          double sigma8, fs2_power** out);
+/* py/py_power.cpp:42-50 py_power_free (the capsule destructor). */
 void fs2_power_free(fs2_power* ps);
 /* py/py_power.cpp:66-77 _power_n */
 int fs2_power_n(const fs2_power* ps, int* n);
@@ -136,7 +143,9 @@ int fs2_particles_len(const fs2_particles* p, uint64_t* np_local);
 int fs2_particles_np_total(const fs2_particles* p, uint64_t* np_total);
 /* py/particle.cpp:35-41 update_np_total (one rank: np_total = np_local) */
 int fs2_particles_update_np_total(fs2_particles* p);
+/* Particles::np_allocated (py/particle.h:36) */
 int fs2_particles_np_allocated(const fs2_particles* p, uint64_t* np_alloc);
+/* Particles::boxsize (py/particle.h:38) */
 int fs2_particles_boxsize(const fs2_particles* p, double* boxsize);
 /* py/py_particles.cpp:403-462 _particles_append: n rows of FP64 x,y,z at
  * x[i*row_stride + c*col_stride] (strides in elements).  Other fields of the
@@ -166,28 +175,36 @@ int fs2_particles_periodic_wrapup(fs2_particles* p);
 /* Particles::a_x, a_v, a_f (py/particle.h:31) */
 int fs2_particles_get_times(const fs2_particles* p, double* a_x, double* a_v,
                             double* a_f);
+/* Sets Particles::a_x, a_v, a_f (py/particle.h:33), as py/lpt.cpp and
+ * py/cola.cpp assign them. */
 int fs2_particles_set_times(fs2_particles* p, double a_x, double a_v, double a_f);
-/* Device planes [3][stride] of each field and the id array (for callers
- * that launch their own work, e.g. autograd).  Applies pending wraps and
+/* Device planes [3][stride] of each field and the id array (no reference
+ * counterpart; for callers that launch their own work, e.g. autograd).  Applies pending wraps and
  * kicks; from then on fs2_cola_kick on these particles runs at once. */
 int fs2_particles_device_fields(fs2_particles* p, float** x, float** v,
                                 float** dx1, float** dx2, float** force,
                                 uint64_t** id, uint64_t* stride);
-/* Device-to-device copy of all state of src into dst (same np_alloc). */
+/* Device-to-device copy of all state of src into dst (same np_alloc; no
+ * reference counterpart). */
 int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src);
 
 /* ---- FFT grids ---------------------------------------------------------- */
 /* py/py_fft.cpp:10-30 _fft_alloc(nc): owning padded real/complex grid */
 int fs2_grid_alloc(int nc, fs2_grid** out);
+/* py/fft.cpp:68-78 FFT::~FFT (the _FFT capsule destructor). */
 void fs2_grid_free(fs2_grid* g);
+/* FFT::nc (py/fft.h:27) */
 int fs2_grid_nc(const fs2_grid* g, int* nc);
 /* py/py_fft.cpp:57-85 _fft_set_test_data: fx[ix,iy,iz] = index + 1 */
 int fs2_grid_set_test_data(fs2_grid* g);
 /* py/py_fft.cpp:87-188 _fft_fx_global_as_array: (nc,nc,nc) FP32 copy */
 int fs2_grid_get_real(fs2_grid* g, float* out);
+/* Host -> device real grid (no reference counterpart; stages test data). */
 int fs2_grid_set_real(fs2_grid* g, const float* in);
-/* Half-complex view nc*nc*(nc/2+1) complex64 in [kx][ky][kz] order. */
+/* Half-complex view nc*nc*(nc/2+1) complex64 in [kx][ky][kz] order, the
+ * layout of FFT::fk (py/fft.h:29); no reference accessor. */
 int fs2_grid_get_complex(fs2_grid* g, float* out);
+/* Host -> device half-complex grid (no reference counterpart). */
 int fs2_grid_set_complex(fs2_grid* g, const float* in);
 /* py/fft.cpp:80-89 execute_forward (r2c, unnormalised, sign -1) */
 int fs2_grid_r2c(fs2_grid* g);
@@ -199,7 +216,11 @@ int fs2_grid_c2r(fs2_grid* g);
 int fs2_pm_create(int nc_pm, double pm_factor, double boxsize, fs2_pm** out);
 /* py/pm.cpp:270-274 pm_free */
 void fs2_pm_destroy(fs2_pm* pm);
+/* The parameters of fs2_pm_create (py/pm.cpp:233-238 keeps them for the
+ * same-parameters check of pm_init). */
 int fs2_pm_params(const fs2_pm* pm, int* nc, double* pm_factor, double* boxsize);
+/* Paint variant FS2_PAINT_* (no reference counterpart: the reference has
+ * one paint loop, py/pm.cpp:57-139, which FS2_PAINT_GLOBAL follows). */
 int fs2_pm_set_paint_variant(fs2_pm* pm, int variant);
 /* py/pm_domain.cpp:94-126 pm_domain_send_positions: the periodic wrap;
  * fused into the next paint, or run on its own before any other use of x. */
@@ -237,13 +258,14 @@ int fs2_cola_drift(fs2_particles* p, double a_pos);
 int fs2_cola_velocity(fs2_particles* p, float* out);
 
 /* ---- named allocations ------------------------------------------------- */
+typedef struct fs2_mem fs2_mem;
 /* py/py_mem.cpp:8-31 _mem_alloc / py/mem.cpp:28-75 Mem(name, size): `size`
  * bytes (rounded up to 16) of device memory, nothing for size 0;
  * FS2_ERR_MEMORY when it does not fit. */
-typedef struct fs2_mem fs2_mem;
 int fs2_mem_alloc(const char* name, uint64_t size, fs2_mem** out);
 /* py/py_mem.cpp:42-53 _mem: (size_alloc, size_using) */
 int fs2_mem_info(const fs2_mem* m, uint64_t* size_alloc, uint64_t* size_using);
+/* py/py_mem.cpp:35-41 py_mem_free (the _Mem capsule destructor). */
 void fs2_mem_free(fs2_mem* m);
 
 /* ---- measurement --------------------------------------------------------- */
@@ -279,7 +301,9 @@ typedef struct fs2_kdtree_node {
   float left, right;
   int32_t ibegin, iend;
 } fs2_kdtree_node;
+/* py/py_kdtree.cpp:36-44 _kdtree_get_height (py/kdtree.cpp kdtree_get_height). */
 int fs2_kdtree_height(uint64_t* height);
+/* py/py_kdtree.cpp:6-34 _kdtree_create_copy: the nodes of the last tree. */
 int fs2_kdtree_nodes(fs2_kdtree_node* out, uint64_t* n);
 
 /* ---- output ------------------------------------------------------------------ */
@@ -387,8 +411,11 @@ int fs2_batch_get_field(fs2_batch* b, int cls, int box, int field, float* out);
  * every kernel launch is bracketed by HIP events on the library stream and
  * accumulated per kernel name with its algorithmic HBM bytes per launch. */
 int fs2_profile_enable(int on);
+/* No reference counterpart. */
 int fs2_profile_reset(void);
+/* No reference counterpart. */
 int fs2_profile_count(int* n);
+/* No reference counterpart. */
 int fs2_profile_get(int i, const char** name, uint64_t* launches,
                     double* total_ms, double* bytes_per_launch);
 
