@@ -172,7 +172,7 @@ int fs2_particles_get_records(fs2_particles* p, uint64_t start, uint64_t count,
                               void* out);
 /* py/py_particles.cpp:371-384 _particles_periodic_wrapup (py/util.h:14-24) */
 int fs2_particles_periodic_wrapup(fs2_particles* p);
-/* Particles::a_x, a_v, a_f (py/particle.h:31) */
+/* Particles::a_x, a_v, a_f (py/particle.h:33) */
 int fs2_particles_get_times(const fs2_particles* p, double* a_x, double* a_v,
                             double* a_f);
 /* Sets Particles::a_x, a_v, a_f (py/particle.h:33), as py/lpt.cpp and
