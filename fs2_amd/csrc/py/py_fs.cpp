@@ -922,7 +922,7 @@ PyObject* py_hdf5_write_particles(PyObject*, PyObject* args) {
 }
 
 // ---------------------------------------------------------------- leapfrog
-// py/py_leapfrog.cpp:4-66
+// py/py_leapfrog.cpp:4-65
 PyObject* py_leapfrog_initial_velocity(PyObject*, PyObject* args) {
   PyObject* o;
   double a;

@@ -301,7 +301,7 @@ typedef struct fs2_kdtree_node {
   float left, right;
   int32_t ibegin, iend;
 } fs2_kdtree_node;
-/* py/py_kdtree.cpp:36-44 _kdtree_get_height (py/kdtree.cpp kdtree_get_height). */
+/* py/py_kdtree.cpp:36-39 _kdtree_get_height (py/kdtree.cpp kdtree_get_height). */
 int fs2_kdtree_height(uint64_t* height);
 /* py/py_kdtree.cpp:6-34 _kdtree_create_copy: the nodes of the last tree. */
 int fs2_kdtree_nodes(fs2_kdtree_node* out, uint64_t* n);

@@ -18,7 +18,7 @@ checks of fs2_fof_find_groups:
   the left one).
 
 Known answers pinned in tests/test_oracle_fof.py: the line cases of the
-reference's test/test_fof.py:40-80 and closed-form small configurations.
+reference's test/test_fof.py:40-74 and closed-form small configurations.
 """
 import numpy as np
 

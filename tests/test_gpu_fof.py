@@ -48,7 +48,7 @@ def _check_reordered(before, after):
 
 @pytest.mark.parametrize("quota", [16, 2])
 def test_line_cases_of_reference_tests(fs, quota):
-    """test/test_fof.py:40-80: a line spaced 1.1 ll is 10 groups, 0.99 ll one."""
+    """test/test_fof.py:40-74: a line spaced 1.1 ll is 10 groups, 0.99 ll one."""
     _announce(f"fof line cases quota={quota}")
     nc, box = 4, 1.0
     dx = box / nc

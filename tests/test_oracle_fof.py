@@ -1,5 +1,5 @@
 """The FoF oracle against known answers: the line cases of the reference's
-test/test_fof.py:40-80, periodic linking across the boundary, a literal
+test/test_fof.py:40-74, periodic linking across the boundary, a literal
 restatement of the breadth-first growth of py/fof.cpp:78-96 without a tree,
 and a hand-derived kd-tree layout (py/kdtree.cpp:94-125)."""
 from collections import deque
