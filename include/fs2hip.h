@@ -324,7 +324,7 @@ int fs2_leapfrog_initial_velocity(fs2_particles* p, double a);
 /* py/leapfrog.cpp:40-68 / py_leapfrog.cpp:24-44 _leapfrog_kick:
  * v += -1.5 Om f SphiStd(a_v, a_vel) */
 int fs2_leapfrog_kick(fs2_particles* p, double a_vel);
-/* py/leapfrog.cpp:70-94 / py_leapfrog.cpp:46-66 _leapfrog_drift:
+/* py/leapfrog.cpp:70-94 / py_leapfrog.cpp:43-61 _leapfrog_drift:
  * x += v SqStd(a_x, a_pos) */
 int fs2_leapfrog_drift(fs2_particles* p, double a_pos);
 

@@ -134,7 +134,7 @@ def test_power_file_errors_raise_ioerror(fs, tmp_path):
         fs.PowerSpectrum(str(tmp_path / "missing.dat"))
     bad = tmp_path / "bad.dat"
     bad.write_text("0.1 1.0\n0.05 2.0\n0.2 3.0\n")
-    with pytest.raises(IOError):   # k must increase (power.cpp:433-439)
+    with pytest.raises(IOError):   # k must increase (power.cpp:108-116)
         fs.PowerSpectrum(str(bad))
 
 
