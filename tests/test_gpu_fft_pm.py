@@ -239,6 +239,32 @@ def test_empty_particle_set(fs, variant):
         fs.pm.set_paint_variant("sorted")
 
 
+def test_partially_filled_particle_set(fs):
+    """np_local < np_alloc: only the appended particles are painted, read out,
+    kicked and drifted (the reference loops to np_local)."""
+    _announce("partially filled particle set")
+    L, nc, n = 64.0, 32, 3000
+    rng = np.random.default_rng(5)
+    fs.cosmology.init(0.31)
+    fs.pm.init(nc, 2.0, L)
+    fs.pm.set_paint_variant("sorted")
+    p = fs.Particles(5000, L)
+    p.append(rng.uniform(0, L, (n, 3)))
+    p.set_times(0.5, 0.5, 0.0)
+    fs.pm.force(p)
+    got = p.force.astype(np.float64)
+    assert got.shape == (n, 3)
+    ref = opm.force(p.x.astype(np.float64), nc, L, 2.0)
+    rel = float(np.sqrt(np.mean((got - ref) ** 2) / np.mean(ref ** 2)))
+    assert rel < 1e-5, rel
+    fs.cola.kick(p, 0.55)
+    fs.cola.drift(p, 0.6)
+    assert len(p) == n and np.all(np.isfinite(p.x)) and np.all(np.isfinite(p.v))
+    fs.pm.send_positions(p)
+    d = fs.pm.compute_density(p).asarray().astype(np.float64)
+    assert abs(d.sum() - (n * 8.0 - nc**3)) < 10 * 1.2e-7 * nc**3   # exactly n particles
+
+
 def _clustered(npart, L, rng):
     """Half uniform, half in a few tight clumps, plus particles on tile and
     box faces (lower cells on the last layer of a tile spill into the next)."""
