@@ -10,6 +10,7 @@ Objects go to fs2_amd/_build/ (git-ignored); the two .so files are written
 in place so that they travel with the source tree.
 """
 import concurrent.futures as cf
+import hashlib
 import os
 import subprocess
 import sys
@@ -24,6 +25,7 @@ LIB = os.path.join(HERE, 'libfs2hip.so')
 IOLIB = os.path.join(HERE, 'libfs2io.so')
 HDF5_DIR = os.environ.get('HDF5_DIR', '/opt/conda')
 PYEXT = os.path.join(HERE, 'fs', '_fs.so')
+STAMP = os.path.join(HERE, 'libfs2hip.so.stamp')
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = 'gfx950'
@@ -95,16 +97,46 @@ def build_pyext():
     return PYEXT
 
 
+def source_hash():
+    """SHA-256 over every file under csrc/ and include/, this recipe, the
+    compiler flags and the HDF5 location: what the built libraries depend on."""
+    h = hashlib.sha256()
+    h.update(repr((HIP_FLAGS, SOURCES, HDF5_DIR, ARCH)).encode())
+    files = [os.path.abspath(__file__)]
+    for top in (CSRC, INCLUDE):
+        for d, _, names in os.walk(top):
+            files += [os.path.join(d, n) for n in names]
+    for f in sorted(files):
+        h.update(os.path.relpath(f, ROOT).encode())
+        with open(f, 'rb') as fp:
+            h.update(fp.read())
+    return h.hexdigest()
+
+
 def build_all(jobs=8):
+    digest = source_hash()
     build_lib(jobs)
     build_iolib()
     build_pyext()
+    with open(STAMP, 'w') as f:
+        f.write(digest + '\n')
+
+
+def is_current():
+    """True when both libraries exist and were built from the sources as
+    they are now (the stamp next to them holds the same source hash)."""
+    if not (os.path.exists(LIB) and os.path.exists(PYEXT) and os.path.exists(STAMP)):
+        return False
+    with open(STAMP) as f:
+        return f.read().strip() == source_hash()
 
 
 def ensure_built(jobs=8):
-    """Builds only when libfs2hip.so or the _fs module is missing (a fresh
-    checkout); an existing build is loaded as it is."""
-    if not (os.path.exists(LIB) and os.path.exists(PYEXT)):
+    """Builds when the libraries are missing or were built from other
+    sources or flags; a current build is loaded as it is."""
+    if not is_current():
+        print('fs2_amd: building libfs2hip.so and fs._fs (sources changed or no build)',
+              file=sys.stderr, flush=True)
         build_all(jobs)
 
 

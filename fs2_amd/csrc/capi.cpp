@@ -869,8 +869,11 @@ int fs2_pm_compute_force(fs2_pm* pm, fs2_particles* p) {
     pm->mesh_is_k = false;
     pm->status = PM_FORCE_DONE;   // the density mesh no longer holds delta
     flush_wrap(p);
+    // The paint's bin records hold the positions it binned.  They are reused
+    // only when x cannot have changed since: not after the device planes were
+    // handed out (fs2_particles_device_fields), whose writes bump no version.
     if (pm->binned_version != 0 && pm->binned_version == p->x_version &&
-        pm->binned_np == p->np_local) {
+        pm->binned_np == p->np_local && !p->planes_exposed) {
       readout_tiled(pm->force[0], pm->force[1], pm->force[2], p->f(), (int64_t)p->np_alloc,
                     (int64_t)p->np_local, n, pm->boxsize_f, pm->paint);
     } else {

@@ -13,9 +13,17 @@ Green x gradient (pm.cpp:460-504):
 with indices <= n/2 mapped to +index (Nyquist is +n/2, pm.cpp:483, 489) and
 kz running over 0..n/2.
 """
-import numpy as np
+import os
 
-from . import cic
+import numpy as np
+from scipy import fft as sfft
+
+from . import cic, cfast
+
+
+def _workers():
+    """Threads for scipy's pocketfft (FP64, the routine numpy.fft wraps)."""
+    return max(1, min(16, int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))))
 
 
 def int_kvectors_pm(n):
@@ -34,7 +42,7 @@ def force_meshes(delta, boxsize, progress=None):
     """Three real force meshes f_a = irfft(i f2 k_a delta_k) (pm.cpp:460-508).
     progress(msg), if given, is called after each transform."""
     n = delta.shape[0]
-    dk = np.fft.rfftn(delta)
+    dk = sfft.rfftn(delta, workers=_workers())
     if progress:
         progress("r2c done")
     kxy, kz = int_kvectors_pm(n)
@@ -49,7 +57,8 @@ def force_meshes(delta, boxsize, progress=None):
         f2 = f1 / k2 * K
         fk = 1j * f2 * dk
         fk[0, 0, 0] = 0.0
-        out.append(np.fft.irfftn(fk, s=(n, n, n), axes=(0, 1, 2)) * n**3)
+        out.append(sfft.irfftn(fk, s=(n, n, n), axes=(0, 1, 2), workers=_workers()) * n**3)
+        del fk
         if progress:
             progress(f"c2r {len(out)} of 3 done")
     return out
@@ -88,3 +97,46 @@ def force(x, nc, boxsize, pm_factor):
     delta = density(x, nc, boxsize, pm_factor)
     meshes = force_meshes(delta, boxsize)
     return cic.readout(x, meshes, boxsize)
+
+
+def force_large(x, nc, boxsize, pm_factor, progress=None, delta=None):
+    """force() for large meshes: the C restatement of the CIC paint and
+    readout (oracle/cic_fp64.c), threaded FP64 FFTs, and one force mesh
+    alive at a time.  delta, if given, is the density of x (from
+    cfast.paint).  Also returns the FP64 sum of delta."""
+    n = nc
+    if delta is None:
+        delta = cfast.paint(x, nc, boxsize, pm_factor)
+        if progress:
+            progress("paint done")
+    total = float(delta.sum())
+    dk = sfft.rfftn(delta, workers=_workers())
+    del delta
+    if progress:
+        progress("r2c done")
+    kxy, kz = int_kvectors_pm(n)
+    f1 = -1.0 / n**3 / (2.0 * np.pi / boxsize)
+    out = np.zeros((len(x), 3), dtype=np.float64)
+    for axis in range(3):
+        fk = np.empty_like(dk)
+        for ix in range(n):   # slab by slab keeps the temporaries small
+            KX = kxy[ix]
+            KY = kxy[:, None]
+            KZ = kz[None, :]
+            k2 = KX * KX + KY * KY + KZ * KZ
+            if ix == 0:
+                k2[0, 0] = 1.0
+            K = (KX + 0 * k2, KY + 0 * k2, KZ + 0 * k2)[axis]
+            fk[ix] = 1j * (f1 / k2 * K) * dk[ix]
+            if ix == 0:
+                fk[0, 0, 0] = 0.0
+        mesh = sfft.irfftn(fk, s=(n, n, n), axes=(0, 1, 2), workers=_workers())
+        del fk
+        mesh *= n**3
+        ms = [None, None, None]
+        ms[axis] = mesh
+        out[:, axis] = cfast.readout(x, ms, boxsize)[:, axis]
+        del mesh, ms
+        if progress:
+            progress(f"axis {axis} done")
+    return out, total

@@ -62,19 +62,41 @@ class TablePower:
         return np.sqrt(fac * s2)
 
 
-def eh98_nowiggle_T(k, omega_m, omega_b, h, tcmb=2.7255):
+# The numerical constants of Eisenstein & Hu 1998 eqs. 26-31, by equation.
+# Tests plant mistakes in a copy of this table to show that the pins fail.
+EH98_CONSTANTS = {
+    's': (44.5, 9.83, 10.0, 0.75),             # eq 26: s = a ln(b/Om h^2)/sqrt(1 + c (Ob h^2)^d)
+    'alpha': (0.328, 431.0, 0.38, 22.3),       # eq 31
+    'gamma': (0.43, 4.0),                      # eq 30
+    'T0': (2.0, 1.8, 14.2, 731.0, 62.5),       # eq 29
+}
+
+
+def eh98_nowiggle_T(k, omega_m, omega_b, h, tcmb=2.7255, constants=None):
+    c = EH98_CONSTANTS if constants is None else constants
+    s1, s2, s3, s4 = c['s']
+    a1, a2, a3, a4 = c['alpha']
+    g1, g2 = c['gamma']
+    t1, t2, t3, t4, t5 = c['T0']
     theta = tcmb / 2.7
     omh2 = omega_m * h * h
     obh2 = omega_b * h * h
     fb = omega_b / omega_m
-    s = 44.5 * np.log(9.83 / omh2) / np.sqrt(1.0 + 10.0 * obh2**0.75)      # eq 26
-    alpha = 1.0 - 0.328 * np.log(431.0 * omh2) * fb + 0.38 * np.log(22.3 * omh2) * fb * fb  # eq 31
+    s = s1 * np.log(s2 / omh2) / np.sqrt(1.0 + s3 * obh2**s4)                  # eq 26
+    alpha = 1.0 - a1 * np.log(a2 * omh2) * fb + a3 * np.log(a4 * omh2) * fb * fb  # eq 31
     kmpc = k * h
-    gamma = omega_m * h * (alpha + (1.0 - alpha) / (1.0 + (0.43 * kmpc * s) ** 4))  # eq 30
+    gamma = omega_m * h * (alpha + (1.0 - alpha) / (1.0 + (g1 * kmpc * s) ** g2))  # eq 30
     q = k * theta * theta / gamma                                              # eq 28
-    L0 = np.log(2.0 * np.e + 1.8 * q)                                          # eq 29
-    C0 = 14.2 + 731.0 / (1.0 + 62.5 * q)
+    L0 = np.log(t1 * np.e + t2 * q)                                            # eq 29
+    C0 = t3 + t4 / (1.0 + t5 * q)
     return L0 / (L0 + C0 * q * q)
+
+
+def eh98_sound_horizon(omega_m, omega_b, h, constants=None):
+    """Eq. 26, the fit to the sound horizon at the drag epoch, in Mpc."""
+    s1, s2, s3, s4 = (EH98_CONSTANTS if constants is None else constants)['s']
+    omh2, obh2 = omega_m * h * h, omega_b * h * h
+    return s1 * np.log(s2 / omh2) / np.sqrt(1.0 + s3 * obh2**s4)
 
 
 def tophat_sigma2(Pfun, R):
@@ -98,14 +120,15 @@ def tophat_sigma2(Pfun, R):
 
 class EH98NoWiggle:
     def __init__(self, omega_m=0.31, omega_b=0.049, h=0.67, ns=0.96, sigma8=0.81,
-                 tcmb=2.7255):
+                 tcmb=2.7255, constants=None):
         self.omega_m, self.omega_b, self.h, self.ns, self.tcmb = omega_m, omega_b, h, ns, tcmb
+        self.constants = constants
         self.A = 1.0
         s2 = tophat_sigma2(self._unnorm, 8.0)
         self.A = sigma8**2 / s2
 
     def _unnorm(self, k):
-        T = eh98_nowiggle_T(k, self.omega_m, self.omega_b, self.h, self.tcmb)
+        T = eh98_nowiggle_T(k, self.omega_m, self.omega_b, self.h, self.tcmb, self.constants)
         return k**self.ns * T * T
 
     def P(self, k):
