@@ -1,6 +1,8 @@
 """bench.py end to end on the GPU at a small size: one JSON line with the
-fields of the bench contract, a roofline for the dominant kernel and
-per-kernel rows whose algorithmic bytes add up to the step total."""
+fields of the bench contract, a roofline for the dominant kernel, the step
+roofline on the compulsory 156 Np + 32 Ng + 32 Nk bytes, per-stage rows that
+add up to it, and per-kernel rows whose declared bytes add up to the
+issued-bytes figure."""
 import json
 import os
 import subprocess
@@ -37,4 +39,16 @@ def test_bench_line_at_small_size():
     # per-kernel GB are rounded to 3 decimals: half a MB per launch at most
     step = sum(k["algorithmic_GB"] * k["launches"] for k in ks.values()) / d["steps"] * 1e9
     slack = sum(k["launches"] for k in ks.values()) / d["steps"] * 0.5e6
-    assert abs(step - d["step_roofline"]["algorithmic_bytes"]) <= slack + 1.0
+    sr = d["step_roofline"]
+    assert abs(step - sr["issued_bytes"]) <= slack + 1.0
+    Np, n = 32**3, 64
+    Ng, Nk = n**3, n * n * (n // 2 + 1)
+    assert sr["algorithmic_bytes"] == 156 * Np + 32 * Ng + 32 * Nk
+    st = d["stage_roofline"]
+    assert set(st) == {"paint", "fft", "readout", "kick_drift"}
+    assert sum(v["algorithmic_bytes"] for v in st.values()) == sr["algorithmic_bytes"]
+    assert all(v["ms"] > 0 and 0 < v["frac"] for v in st.values())
+    # the stages hold every kernel of the step but the bookkeeping ones
+    stage_ms = sum(v["ms"] for v in st.values())
+    kern_ms = sum(k["total_ms"] for k in ks.values()) / d["steps"]
+    assert stage_ms <= kern_ms + 1e-3 and stage_ms > 0.8 * kern_ms

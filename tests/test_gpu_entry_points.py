@@ -169,19 +169,30 @@ def test_positions_written_through_device_pointers_are_read_out(fs, eh):
 
 
 def test_linear_regime_keeps_the_lpt_trajectory(fs, tmp_path):
-    """A tiny, smooth (k < 0.15 h/Mpc) spectrum: PM forces equal the
-    linear force -D1 dx1, so the 10 COLA steps keep x on
-    q + D1(a) dx1 + D2(a) dx2, the COLA residual velocity stays near zero
-    and cola.velocity comes out as Dv(a) dx1 (+ the D2v term)."""
+    """A tiny, smooth (k < 0.15 h/Mpc) spectrum: the PM force is close to the
+    linear force -D1 dx1, so the 10 COLA steps keep x near
+    q + D1(a) dx1 + D2(a) dx2, the COLA residual velocity stays small against
+    Dv dx1, and cola.velocity comes out near Dv(a) dx1 (+ the D2v term).
+
+    Mesh = particle grid (pm_factor 1, as script/cola.py): the particles sit
+    at cell centres.  The FP64 oracle gives deviations of 3.3% (x) and 4.9%
+    (v) for this case, the CIC window and grid discreteness at k ~ k_f; a
+    wrong sign or factor in the force, kick or drift gives order one.  (With
+    pm_factor 2 and the default lattice offset of 0.5 the particles sit on
+    mesh nodes, where the CIC kink makes the force response to displacements
+    this small non-smooth: the oracle's deviation is then 83%, and so is the
+    GPU's.)"""
     _announce("linear regime invariant")
     fs.cosmology.init(OM)
     c = ocosmo.Cosmology(OM)
-    k = np.logspace(-2, np.log10(5.0), 400)
+    # the table spans the modes of the box (0.098 to 2.6 h/Mpc) with P > 0
+    # throughout (ln P is splined): exp(-(3/0.15)^2) = 2e-174
+    k = np.logspace(-2, np.log10(3.0), 400)
     P = 10.0 * k * np.exp(-(k / 0.15) ** 2)
     path = os.path.join(tmp_path, "smooth_pk.dat")
     np.savetxt(path, np.stack([k, P], 1))
     ps = fs.PowerSpectrum(path)
-    n_p, nc, L = 32, 64, 64.0
+    n_p, nc, L = 32, 32, 64.0
     p = fs.lpt.init(n_p, L, 0.1, ps, 23, "cola")
     dx1, dx2 = p.dx1.astype(np.float64), p.dx2.astype(np.float64)
     q = p.x.astype(np.float64) - c.D(0.1) * dx1 - c.D2(0.1) * dx2
@@ -203,4 +214,4 @@ def test_linear_regime_keeps_the_lpt_trajectory(fs, tmp_path):
     print(f"[case] linear regime: disp {disp:.3e} Mpc/h, x dev {rx:.2e}, v_cola {rv:.2e}, "
           f"velocity dev {rc:.2e}", flush=True)
     assert disp > 1e-2
-    assert rx < 2e-2 and rv < 2e-2 and rc < 2e-2
+    assert rx < 0.07 and rv < 0.1 and rc < 0.1
