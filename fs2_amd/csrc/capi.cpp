@@ -106,6 +106,7 @@ void device_init() {
          hipGetErrorString(e), n);
   FS2_HIP(hipSetDevice(0));
   FS2_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  pk_scratch_init();   // device 0 is current
   g_inited = true;
 }
 
@@ -774,6 +775,7 @@ int fs2_pm_create(int nc_pm, double pm_factor, double boxsize, fs2_pm** out) {
     try {
       pm->mesh = (float*)dmalloc(bytes);
       for (int a = 0; a < 3; ++a) pm->force[a] = (float*)dmalloc(bytes);
+      fft_prepare(nc_pm);   // the mesh's transforms never allocate after this
     } catch (...) {
       dfree(pm->mesh);
       for (int a = 0; a < 3; ++a) dfree(pm->force[a]);
@@ -912,6 +914,22 @@ int fs2_pm_power_spectrum(fs2_pm* pm, int nbins, double dk_bin, double* k, doubl
       pm->mesh_is_k = true;
     }
     power_spectrum(pm->mesh, pm->nc, pm->boxsize, nbins, dk_bin, k, P, nmodes);
+  });
+}
+
+int fs2_pm_power_spectrum_noalloc(fs2_pm* pm, int nbins, double dk_bin, double* k, double* P,
+                                  double* nmodes) {
+  return guard([&] {
+    if (nbins > FS2_PK_NOALLOC_MAX_BINS)
+      fail(FS2_ERR_VALUE, "%d P(k) bins exceed the fixed scratch of %d bins", nbins,
+           FS2_PK_NOALLOC_MAX_BINS);
+    if (pm->status != PM_DENSITY_DONE && !pm->mesh_is_k)
+      fail(FS2_ERR_RUNTIME, "no density mesh for the power spectrum");
+    if (!pm->mesh_is_k) {
+      fft_r2c_3d(pm->mesh, pm->nc);   // twiddles built by fs2_pm_create
+      pm->mesh_is_k = true;
+    }
+    power_spectrum_noalloc(pm->mesh, pm->nc, pm->boxsize, nbins, dk_bin, k, P, nmodes);
   });
 }
 

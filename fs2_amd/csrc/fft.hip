@@ -972,6 +972,13 @@ bool fft_supported(int n) {
   }
 }
 
+void fft_prepare(int n) {
+  dispatch(n, [&](auto impl) {
+    twiddles(decltype(impl)::N);
+    twiddles(decltype(impl)::M);
+  });
+}
+
 void fft_r2c_3d(float* grid, int n, int nb) {
   dispatch(n, [&](auto impl) { decltype(impl)::r2c(grid, nb); });
 }

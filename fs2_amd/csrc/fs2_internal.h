@@ -202,6 +202,16 @@ void grad_readout_tiled(const float* rhobar, float fac, const float* const F[3],
                         float boxsize, const PaintScratch& s, const TileOpts& o = TileOpts());
 void power_spectrum(const float* delta_k, int n, double boxsize, int nbins,
                     double dk_bin, double* k, double* P, double* nmodes);
+// The same without any allocation: the sums go to the per-GPU `scratch`
+// array of pm_kernels.hip (nbins <= FS2_PK_NOALLOC_MAX_BINS).
+void power_spectrum_noalloc(const float* delta_k, int n, double boxsize, int nbins,
+                            double dk_bin, double* k, double* P, double* nmodes);
+// Records the device address of that array for the current device; called
+// once per device at library set-up, with the device current.
+void pk_scratch_init();
+// Builds the twiddle tables of an n^3 mesh's passes, so later transforms of
+// that size allocate nothing.
+void fft_prepare(int n);
 
 // ---- COLA kernels (cola_kernels.hip) --------------------------------------------
 void cola_kick(float* v, const float* f, const float* dx1, const float* dx2,

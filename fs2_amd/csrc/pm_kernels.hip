@@ -12,10 +12,20 @@
 
 #pragma clang fp contract(off)
 
+// Bin sums of the allocation-free P(k) entry point
+// (fs2_pm_power_spectrum_noalloc): counts, k sums and P sums of up to
+// FS2_PK_NOALLOC_MAX_BINS bins, FP64.  Each GPU holds its own copy for the
+// life of the process.  In host code this name is only a stand-in for the
+// device address: host code uses the pointer that pk_scratch_init obtains
+// with hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[3 * FS2_PK_NOALLOC_MAX_BINS];
+
 namespace fs2 {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMaxDevices = 64;
+double* g_pk_scratch[kMaxDevices];   // device address of `scratch`, per device
 
 inline unsigned blocks_for(int64_t n, int per_block = kBlock) {
   return (unsigned)((n + per_block - 1) / per_block);
@@ -181,7 +191,52 @@ __global__ void k_power_spectrum(const float2* __restrict__ dk, int n, int cpitc
     if (bins[b] != 0.0) atomicAdd(&out[b], bins[b]);
 }
 
+__global__ void k_zero_f64(double* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// Clears the 3*nbins sums with a kernel (no memset node, so the sequence can
+// be captured), bins them, copies counts, k sums and P sums into the caller's
+// arrays and normalises those in place: no host or device allocation.
+void power_spectrum_sums(const float* delta_k, int n, double boxsize, int nbins, double dk_bin,
+                         double* d, double* k, double* P, double* nmodes) {
+  launch("pk_clear", (double)nbins * 24.0, k_zero_f64, dim3((3 * nbins + kBlock - 1) / kBlock),
+         dim3(kBlock), 0, d, 3 * nbins);
+  const double ng = (double)n * n * n;
+  const double pnorm = boxsize * boxsize * boxsize / (ng * ng);
+  const int64_t total = (int64_t)n * n * (n / 2 + 1);
+  const unsigned nb = (unsigned)std::min<int64_t>(blocks_for(total), 2048);
+  launch("power_spectrum", (double)total * 8.0, k_power_spectrum, dim3(nb), dim3(kBlock),
+         3 * nbins * sizeof(double), (const float2*)delta_k, n, grid_cpitch(n), nbins, dk_bin,
+         pnorm, d);
+  const size_t bytes = sizeof(double) * nbins;
+  FS2_HIP(hipMemcpyAsync(nmodes, d, bytes, hipMemcpyDeviceToHost, stream()));
+  FS2_HIP(hipMemcpyAsync(k, d + nbins, bytes, hipMemcpyDeviceToHost, stream()));
+  FS2_HIP(hipMemcpyAsync(P, d + 2 * nbins, bytes, hipMemcpyDeviceToHost, stream()));
+  FS2_HIP(hipStreamSynchronize(stream()));
+  const double kf = 2.0 * M_PI / boxsize;
+  for (int b = 0; b < nbins; ++b) {
+    const double c = nmodes[b];
+    k[b] = c > 0 ? k[b] / c * kf : 0.0;
+    P[b] = c > 0 ? P[b] / c : 0.0;
+  }
+}
+
 }  // namespace
+
+void pk_scratch_init() {
+  int dev = 0;
+  FS2_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(FS2_ERR_DEVICE, "device %d out of range", dev);
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    (void)hipGetLastError();
+    fail(FS2_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+  }
+  g_pk_scratch[dev] = (double*)ptr;
+}
 
 void fill_grid(float* grid, int n, float value) {
   const int64_t total = (int64_t)grid_floats(n);
@@ -237,26 +292,26 @@ void power_spectrum(const float* delta_k, int n, double boxsize, int nbins, doub
   if (nbins <= 0) return;
   if (3 * nbins * sizeof(double) > 60000) fail(FS2_ERR_VALUE, "too many P(k) bins: %d", nbins);
   double* d = (double*)dmalloc(sizeof(double) * 3 * nbins);
-  FS2_HIP(hipMemsetAsync(d, 0, sizeof(double) * 3 * nbins, stream()));
-  const double ng = (double)n * n * n;
-  const double pnorm = boxsize * boxsize * boxsize / (ng * ng);
-  const int64_t total = (int64_t)n * n * (n / 2 + 1);
-  const unsigned nb = (unsigned)std::min<int64_t>(blocks_for(total), 2048);
-  launch("power_spectrum", (double)total * 8.0, k_power_spectrum, dim3(nb), dim3(kBlock),
-         3 * nbins * sizeof(double), (const float2*)delta_k, n, grid_cpitch(n), nbins, dk_bin,
-         pnorm, d);
-  std::vector<double> h(3 * nbins);
-  FS2_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * 3 * nbins, hipMemcpyDeviceToHost,
-                         stream()));
-  FS2_HIP(hipStreamSynchronize(stream()));
-  dfree(d);
-  const double kf = 2.0 * M_PI / boxsize;
-  for (int b = 0; b < nbins; ++b) {
-    const double c = h[b];
-    nmodes[b] = c;
-    k[b] = c > 0 ? h[nbins + b] / c * kf : 0.0;
-    P[b] = c > 0 ? h[2 * nbins + b] / c : 0.0;
+  try {
+    power_spectrum_sums(delta_k, n, boxsize, nbins, dk_bin, d, k, P, nmodes);
+  } catch (...) {
+    dfree(d);
+    throw;
   }
+  dfree(d);
+}
+
+void power_spectrum_noalloc(const float* delta_k, int n, double boxsize, int nbins,
+                            double dk_bin, double* k, double* P, double* nmodes) {
+  if (nbins <= 0) return;
+  if (nbins > FS2_PK_NOALLOC_MAX_BINS)
+    fail(FS2_ERR_VALUE, "%d P(k) bins exceed the fixed scratch of %d bins", nbins,
+         FS2_PK_NOALLOC_MAX_BINS);
+  int dev = 0;
+  FS2_HIP(hipGetDevice(&dev));
+  double* d = (dev >= 0 && dev < kMaxDevices) ? g_pk_scratch[dev] : nullptr;
+  if (!d) fail(FS2_ERR_DEVICE, "P(k) scratch not set up on device %d", dev);
+  power_spectrum_sums(delta_k, n, boxsize, nbins, dk_bin, d, k, P, nmodes);
 }
 
 }  // namespace fs2

@@ -245,6 +245,17 @@ int fs2_pm_get_force_mesh(fs2_pm* pm, int axis, float* out);
  * < b+1; outputs nbins values each (k in h/Mpc). */
 int fs2_pm_power_spectrum(fs2_pm* pm, int nbins, double dk_bin, double* k,
                           double* P, double* nmodes);
+/* No counterpart in the reference (which has no P(k) estimator at all):
+ * fs2_pm_power_spectrum without any allocation, for callers that cannot
+ * allocate -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  The FP64 bin sums live in one fixed per-GPU array
+ * of 3 x FS2_PK_NOALLOC_MAX_BINS doubles (48 KiB), held until the process
+ * ends; nbins above FS2_PK_NOALLOC_MAX_BINS gives FS2_ERR_VALUE.  Each call
+ * clears the bins it uses first.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other host threads, other streams) race. */
+#define FS2_PK_NOALLOC_MAX_BINS 2048
+int fs2_pm_power_spectrum_noalloc(fs2_pm* pm, int nbins, double dk_bin, double* k,
+                                  double* P, double* nmodes);
 
 /* ---- COLA ---------------------------------------------------------------- */
 /* py/cola.cpp:48-92 cola_kick.  The coefficients are computed (and checked)
