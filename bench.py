@@ -176,52 +176,96 @@ def fof_leg(fs, ps, n_p=256, nc=512, box=512.0, nstep=10):
     }
 
 
-def cpu_baseline(n_p, nc, box, sub, workers):
-    """Oracle port on the host: one COLA step with the four FFTs at full mesh
-    size (FP32 scipy.fft, `workers` threads) and the particle loops (paint,
-    readout, kick, drift) over a 1/`sub` subsample, scaled back by `sub`."""
-    import numpy as np
-    from oracle import cic as ocic
-    from oracle import cola as ocola
-    from oracle import cosmology as ocosmo
-    from oracle import pm as opm
+def _cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may use,
+    at most 16 (one GPU's share of its host)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return max(1, min(16, usable))
 
-    rng = np.random.default_rng(0)
-    npart = n_p**3 // sub
-    x = rng.uniform(0, box, (npart, 3)).astype(np.float32)
-    v = np.zeros_like(x)
-    dx1 = rng.standard_normal((npart, 3)).astype(np.float32)
-    dx2 = rng.standard_normal((npart, 3)).astype(np.float32)
-    cosmo = ocosmo.Cosmology(0.31)
-    pm_factor = nc / n_p
 
-    t0 = time.perf_counter()
-    delta = ocic.paint(x, nc, box, pm_factor, wdtype=np.float32).astype(np.float32)
-    t_paint = time.perf_counter() - t0
+def _records_of(p):
+    from oracle import cpu_baseline as cb
+    return cb.records(p.x, p.v, p.dx1, p.dx2, p.id)
 
-    t0 = time.perf_counter()
-    meshes = opm.force_meshes_fp32(delta, box, workers=workers)
-    t_fft = time.perf_counter() - t0
-    del delta
 
-    t0 = time.perf_counter()
-    f = ocic.readout(x, meshes, box, wdtype=np.float32).astype(np.float32)
-    del meshes
-    v = ocola.kick(v, f, dx1, dx2, cosmo, 0.1, 0.1, 0.145).astype(np.float32)
-    x = ocola.drift(x, v, dx1, dx2, cosmo, 0.1, 0.145, 0.19).astype(np.float32)
-    t_part = time.perf_counter() - t0
+def _cpu_run(recs, nc, box, pm_factor, a_x, a_v, steps, threads):
+    """Runs `steps` of the reference's CPU step (oracle/cpu_step.cpp) and
+    returns seconds per step and per phase."""
+    from oracle import cpu_baseline as cb
+    sim = cb.CpuCola(recs, nc, box, pm_factor, 0.31, a_x, threads, a_v=a_v)
+    tot, phases = 0.0, {}
+    for a_vel, a_pos in steps:
+        t0 = time.perf_counter()
+        ph = sim.step(a_vel, a_pos)
+        tot += time.perf_counter() - t0
+        for k, v in ph.items():
+            phases[k] = phases.get(k, 0.0) + v
+    n = len(steps)
+    return tot / n, {k: round(v / n, 3) for k, v in phases.items()}, sim.threads
 
-    t_step = t_fft + sub * (t_paint + t_part)
+
+def cpu_baseline(state, n_p, nc, box):
+    """The reference's CPU step (oracle/cpu_step.cpp: the loops of
+    py/pm.cpp and py/cola.cpp on 56-byte AoS records, OpenMP at the
+    reference's pragma sites, FP32 pocketfft transforms) timed for one step
+    on the final, clustered state of the GPU run, with all usable host cores
+    (at most 16)."""
+    from oracle import cpu_baseline as cb
+    threads = _cpu_threads()
+    recs, a_x, a_v = state
+    t_step, phases, used = _cpu_run(recs, nc, box, nc / n_p, a_x, a_v,
+                                    [(a_v + 0.09, a_x + 0.09)], threads)
     return {
         "value": n_p**3 / t_step,
         "unit": "particle-steps/s",
-        "cores": workers,
+        "cores": used,
         "kind": "port",
-        "sample": (f"1 COLA step of the FP32 oracle port at {nc}^3 mesh: 4 FFTs at full size "
-                   f"(scipy.fft, {workers} threads, {t_fft:.1f} s) + paint/readout/kick/drift "
-                   f"over 1/{sub} of the {n_p}^3 particles (numpy, 1 thread, "
-                   f"{t_paint + t_part:.1f} s) scaled x{sub}"),
+        "seconds_per_step": round(t_step, 3),
+        "phases_s": phases,
+        "host": cb.host_info(),
+        "sample": (f"1 COLA step of the reference's CPU loops (C++ port, OpenMP at its pragma "
+                   f"sites, {used} threads; FP32 pocketfft, {used} workers) at {n_p}^3 "
+                   f"particles / {nc}^3 mesh on the a = {a_x:g} state of the GPU run"),
     }
+
+
+def cpu_full_leg(fs, ps):
+    """The CPU baseline planned for the reference's loops: configurations
+    A (64^3 / 128^3, box 64) and B (256^3 / 512^3, box 512) timed over all 10
+    COLA steps from the 2LPT state, C (512^3 / 1024^3, box 1024) for the
+    last 2 steps from the GPU state after 8, extrapolated to 10; each with 1
+    thread (the reference's shipped build has no OpenMP) and with all usable
+    cores (at most 16)."""
+    from oracle import cpu_baseline as cb
+    out = {"host": cb.host_info()}
+    threads = sorted({1, _cpu_threads()})
+    sched = schedule(0.1, 1.0, 10)
+    for name, n_p, nc, box in (("A", 64, 128, 64.0), ("B", 256, 512, 512.0),
+                               ("C", 512, 1024, 1024.0)):
+        p = fs.lpt.init(n_p, box, 0.1, ps, 1, "cola")
+        steps, a_x, a_v = sched, 0.1, 0.1
+        if name == "C":
+            fs.pm.init(nc, nc / n_p, box)
+            fs.pm.set_paint_variant("sorted")
+            for a_vel, a_pos in sched[:8]:
+                cola_step(fs, p, a_vel, a_pos)
+            steps, a_x, a_v = sched[8:], sched[7][1], sched[7][0]
+        recs = _records_of(p)
+        del p
+        for t in threads:
+            print(f"[bench] cpu_full {name} {t} thread(s)", file=sys.stderr, flush=True)
+            t_step, phases, used = _cpu_run(recs.copy(), nc, box, nc / n_p, a_x, a_v, steps, t)
+            out[f"{name}_{used}t"] = {
+                "particles": n_p**3, "mesh": nc, "steps_timed": len(steps),
+                "extrapolated_to_10_steps": len(steps) != 10,
+                "seconds_per_step": round(t_step, 3),
+                "seconds_10_steps": round(10 * t_step, 2),
+                "value": n_p**3 / t_step, "unit": "particle-steps/s", "threads": used,
+                "phases_s": phases}
+    return out
 
 
 PROFILE_TABLE = os.path.join(ROOT, "profiles", "cola_512p_1024mesh_kernels.md")
@@ -347,6 +391,10 @@ def main_leg(args, fs, ps):
         c._synchronize()
         paint_ms[variant] = round((time.perf_counter() - t1) / 3 * 1e3, 3)
     fs.pm.set_paint_variant("sorted")
+    state = None
+    if args.cpu_baseline == "on":
+        a_x, a_v, _ = p.times
+        state = (_records_of(p), a_x, a_v)
     del p
 
     npart = args.np**3
@@ -390,6 +438,7 @@ def main_leg(args, fs, ps):
         "stage_roofline": stages,
         "kernels": kernels,
         "paint_variants_ms_final_state": paint_ms,
+        "_state": state,
     }
 
 
@@ -403,12 +452,13 @@ def main():
     ap.add_argument("--box", type=float, default=1024.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
-    ap.add_argument("--cpu-sub", type=int, default=8)
     ap.add_argument("--legs", default="main,adjoint,batch",
                     help="comma list of main (headline COLA box), adjoint (differentiable "
                          "128^3/256^3 run), batch (96-box hipGraph ensemble), fof "
                          "(halo finder after a 256^3 COLA run), adjoint_c (the differentiable run at "
-                         "512^3/1024^3); fof and adjoint_c are not in the default list")
+                         "512^3/1024^3), cpu_full (the reference's CPU step at A, B and C, 1 "
+                         "thread and all cores; minutes); fof, adjoint_c and cpu_full are not in "
+                         "the default list")
     args = ap.parse_args()
     legs = set(x.strip() for x in args.legs.split(",") if x.strip())
 
@@ -454,8 +504,10 @@ def main():
             print(f"[bench] {name} failed: {e!r}", file=sys.stderr, flush=True)
             return {"error": f"{type(e).__name__}: {e}"}
 
+    state = None
     if "main" in legs:
         out.update(main_leg(args, fs, ps))
+        state = out.pop("_state")
         out["copy_peak"] = optional("copy probe", lambda: copy_probe(fs))
         if "GBps" in out["copy_peak"]:
             out["roofline"]["frac_of_copy_peak"] = round(
@@ -474,14 +526,14 @@ def main():
     if "fof" in legs:
         print("[bench] friends-of-friends 256^3", file=sys.stderr, flush=True)
         out["fof"] = optional("fof", lambda: fof_leg(fs, ps))
-    if args.cpu_baseline == "on" and "main" in legs:
+    if "cpu_full" in legs:
+        print("[bench] cpu baseline, configurations A, B, C", file=sys.stderr, flush=True)
+        out["cpu_full"] = optional("cpu full", lambda: cpu_full_leg(fs, ps))
+    if state is not None:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        try:
-            workers = min(16, len(os.sched_getaffinity(0)))
-        except AttributeError:
-            workers = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = optional(
-            "cpu baseline", lambda: cpu_baseline(args.np, args.nc, args.box, args.cpu_sub, workers))
+            "cpu baseline", lambda: cpu_baseline(state, args.np, args.nc, args.box))
+        del state
     print(json.dumps(out), flush=True)
 
 

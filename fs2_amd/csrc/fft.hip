@@ -26,6 +26,7 @@
 #include <mutex>
 
 #include "fs2_internal.h"
+#include "fs2_device.h"
 
 namespace fs2 {
 namespace {

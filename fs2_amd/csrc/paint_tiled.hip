@@ -29,6 +29,7 @@
 #include <type_traits>
 
 #include "fs2_internal.h"
+#include "fs2_device.h"
 
 #pragma clang fp contract(off)
 

@@ -8,22 +8,14 @@ def set_initial_velocity(particles, a_vel):
 
 
 def kick(particles, a_vel):
-    """Update particle velocities to scale factor a_vel
-
-    Args:
-        particles (Particles)
-        a_vel (float): Scale factor after kick.
-    """
+    """Advances the velocities from their current time a_v to a_vel with the
+    forces at a_x: v += -1.5 Omega_m f SphiStd(a_v, a_vel)."""
     c._leapfrog_kick(particles._particles, a_vel)
 
 
 def drift(particles, a_pos):
-    """Update particle positions to scale factor a_pos
-
-    Args:
-        particles (Particles)
-        a_pos (float): Scale factor after drift.
-    """
+    """Advances the positions from a_x to a_pos at the current velocities:
+    x += v SqStd(a_x, a_pos)."""
     c._leapfrog_drift(particles._particles, a_pos)
 
 

@@ -43,28 +43,16 @@ class Particles(object):
         c._particles_periodic_wrapup(self._particles)
 
     def save_gadget_binary(self, filename, use_longid=False):
-        """Gadget-1 binary snapshot (py/fs/particles.py:83-89).
-
-        Args:
-            filename (str): Output file name.
-            use_longid (bool): Write 8-byte ID (default: False)
-        """
+        """Writes a Gadget-1 snapshot: 256-byte header, positions,
+        velocities / sqrt(a), then ids as uint32, or uint64 when use_longid
+        (py/fs/particles.py:83-89)."""
         c._write_gadget_binary(self._particles, filename, use_longid)
 
     def save_hdf5(self, filename, var):
-        """HDF5 snapshot (py/fs/particles.py:91-104).
-
-        Args:
-            filename (str): Output file name.
-            var (str): Output variables, a subset of 'ixvcf12' in this order.
-
-                * i: ID
-                * x: positions
-                * v: velocities (c: COLA velocities, written as v)
-                * f: force
-                * 1: 1LPT displacements (a=1)
-                * 2: 2nd-order displacements (a=1)
-        """
+        """Writes an HDF5 snapshot with /parameters and one dataset per letter
+        of var (py/fs/particles.py:91-104): 'i' ids, 'x' positions, 'v'
+        velocities ('c' writes the COLA-corrected velocity as v), 'f' forces,
+        '1' and '2' the first- and second-order displacements at a = 1."""
         c._hdf5_write_particles(self._particles, filename, var)
 
     def copy_from(self, other):
