@@ -200,7 +200,10 @@ def _cpu_run(recs, nc, box, pm_factor, a_x, a_v, steps, threads):
     for a_vel, a_pos in steps:
         t0 = time.perf_counter()
         ph = sim.step(a_vel, a_pos)
-        tot += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        tot += dt
+        print(f"[bench] cpu step {nc}^3 mesh, {sim.threads} thread(s): {dt:.2f} s",
+              file=sys.stderr, flush=True)
         for k, v in ph.items():
             phases[k] = phases.get(k, 0.0) + v
     n = len(steps)
