@@ -1280,7 +1280,7 @@ int fs2_lpt(int nc, double boxsize, double a, uint64_t seed, const fs2_power* ps
     const std::string k(kind ? kind : "");
     if (k != "cola" && k != "zeldovich" && k != "2lpt")
       fail(FS2_ERR_VALUE, "unknown LPT kind '%s' (zeldovich, 2lpt, cola)", k.c_str());
-    if (nc < 4 || nc % 4 != 0 || !fft_supported(nc))
+    if (nc < 4 || nc % 2 != 0 || !fft_supported(nc))
       fail(FS2_ERR_VALUE, "nc = %d not supported for 2LPT", nc);
     if (boxsize <= 0) fail(FS2_ERR_VALUE, "boxsize must be positive");
     (void)host::omega_m();   // cosmology must be initialised

@@ -954,16 +954,298 @@ void dispatch(int n, F&& f) {
     case 256: f(Impl<256>{}); break;
     case 512: f(Impl<512>{}); break;
     case 1024: f(Impl<1024>{}); break;
-    default: fail(FS2_ERR_VALUE, "FFT size %d not supported (4..1024: 2^k, 48, 96)", n);
+    default: fail(FS2_ERR_VALUE, "FFT size %d not supported: n must be even with prime factors "
+                  "2, 3, 5, 7 (4 <= n <= 4096)", n);
   }
 }
 
-}  // namespace
 
-static_assert(1024LL * 1024 * (((1024 / 2 + 1) + 15) & ~15) < (1LL << 31),
-              "32-bit float2 offsets in the pencil kernels");
+// ---- sizes without a tuned plan --------------------------------------------------
+// Every other even n whose prime factors are 2, 3, 5 and 7 (FFTW takes any
+// n; odd n and larger primes are refused): runtime-planned mixed-radix
+// Stockham transforms in LDS, one kernel per axis, all offsets 64-bit (n up
+// to 4096; a 2048^3 mesh holds 8.7e9 floats).  A workgroup takes P lines
+// (P N <= 4096 values, two LDS buffers of 32 KB).  Stage s of radix R with
+// Ns the product of the earlier radices maps element j + r N/R (r < R) of
+// the source to (j / Ns) Ns R + j % Ns + p Ns of the destination after the
+// twiddles W_N^(r (j % Ns) N / (Ns R)) and a length-R DFT whose factors are
+// also read from the N-entry FP64-built table.  The real transforms along z
+// run an n/2-point complex transform of the packed row with a split (r2c)
+// or merge (c2r) pass; the Green x gradient multiplier is a separate pass.
+// Results equal numpy's rfftn / irfftn conventions, as the tuned passes.
+constexpr int kGenThreads = 256;
+constexpr int kGenValues = 4096;
+constexpr int kGenMaxStages = 24;
 
-bool fft_supported(int n) {
+struct GenPlan {
+  int nst;
+  int R[kGenMaxStages];
+};
+
+template <int R>
+__device__ __forceinline__ void gen_stage(const float2* a, float2* b, int N, int P, int Ns,
+                                          const float2* __restrict__ tw, bool inv) {
+  const int NR = N / R;
+  const int tstep = N / (Ns * R);
+  for (int q = threadIdx.x; q < P * NR; q += blockDim.x) {
+    const int c = q / NR, j = q - c * NR;
+    const int k = j % Ns;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = a[c * N + j + r * NR];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      float2 w = tw[(int)(((int64_t)r * k * tstep) % N)];
+      if (inv) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+    const int idx = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int p = 0; p < R; ++p) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[((p * r) % R) * NR];
+        if (inv) w.y = -w.y;
+        acc = cadd(acc, cmul(v[r], w));
+      }
+      b[c * N + idx + p * Ns] = acc;
+    }
+  }
+}
+
+// Lines (o, c) of nb meshes: element j at data + box*bstride + o*ostride +
+// c*cstride + j*elem; the grid is (nouter * nchunks, nb), chunk = P lines.
+__global__ void __launch_bounds__(kGenThreads)
+k_gen_lines(float2* __restrict__ data, int N, int P, int64_t elem, int64_t ostride,
+            int64_t cstride, int ncols, int nchunks, int64_t bstride,
+            const float2* __restrict__ tw, GenPlan plan, int inv) {
+  extern __shared__ float2 gls[];
+  float2* a = gls;
+  float2* b = gls + P * N;
+  const int o = blockIdx.x / nchunks, ch = blockIdx.x - o * nchunks;
+  const int c0 = ch * P;
+  const int pc = min(P, ncols - c0);
+  float2* d = data + (int64_t)blockIdx.y * bstride + (int64_t)o * ostride + (int64_t)c0 * cstride;
+  const bool rows = elem == 1;   // contiguous lines: consecutive threads along j
+  for (int i = threadIdx.x; i < pc * N; i += blockDim.x) {
+    const int c = rows ? i / N : i % pc, j = rows ? i - (i / N) * N : i / pc;
+    a[c * N + j] = d[(int64_t)c * cstride + (int64_t)j * elem];
+  }
+  __syncthreads();
+  int Ns = 1;
+  for (int s = 0; s < plan.nst; ++s) {
+    const int R = plan.R[s];
+    switch (R) {
+      case 2: gen_stage<2>(a, b, N, pc, Ns, tw, inv); break;
+      case 3: gen_stage<3>(a, b, N, pc, Ns, tw, inv); break;
+      case 4: gen_stage<4>(a, b, N, pc, Ns, tw, inv); break;
+      case 5: gen_stage<5>(a, b, N, pc, Ns, tw, inv); break;
+      case 7: gen_stage<7>(a, b, N, pc, Ns, tw, inv); break;
+      default: gen_stage<8>(a, b, N, pc, Ns, tw, inv); break;
+    }
+    __syncthreads();
+    float2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  for (int i = threadIdx.x; i < pc * N; i += blockDim.x) {
+    const int c = rows ? i / N : i % pc, j = rows ? i - (i / N) * N : i / pc;
+    d[(int64_t)c * cstride + (int64_t)j * elem] = a[c * N + j];
+  }
+}
+
+// r2c split of rows holding Z = FFT_M(x[2m] + i x[2m+1]) in their first M
+// values: X[k] = (Z_k + conj Z_{M-k})/2 - (i/2) W_N^k (Z_k - conj Z_{M-k}),
+// k = 0..M, in place; one work item per pair (k, M-k).
+__global__ void k_gen_split(float2* __restrict__ data, int64_t nrows, int M, int cpitch,
+                            const float2* __restrict__ twN) {
+  const int64_t per = M / 2 + 1;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrows * per;
+       q += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t row = q / per;
+  const int k = (int)(q - row * per);
+  float2* d = data + row * cpitch;
+  if (k == 0) {
+    const float2 z = d[0];
+    d[0] = make_float2(z.x + z.y, 0.f);
+    d[M] = make_float2(z.x - z.y, 0.f);
+    continue;
+  }
+  const int kk = M - k;
+  const float2 zk = d[k], zm = d[kk];
+  auto one = [&](float2 a, float2 b, int kw) {   // (a + conj b)/2 - (i/2) W^kw (a - conj b)
+    const float2 s = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+    const float2 t = cmul(twN[kw], make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y)));
+    return make_float2(s.x + t.y, s.y - t.x);
+  };
+  const float2 xk = one(zk, zm, k);
+  if (kk != k) d[kk] = one(zm, zk, kk);
+  d[k] = xk;
+  }
+}
+
+// c2r merge: Z_k = (X_k + conj X_{M-k}) + i conj(W_N^k) (X_k - conj X_{M-k})
+// for k < M, the imaginary parts of X_0 and X_M dropped (numpy's irfft),
+// so that the inverse M-point transform gives x[2m] + i x[2m+1].
+__global__ void k_gen_merge(float2* __restrict__ data, int64_t nrows, int M, int cpitch,
+                            const float2* __restrict__ twN) {
+  const int64_t per = M / 2 + 1;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrows * per;
+       q += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t row = q / per;
+  const int k = (int)(q - row * per);
+  float2* d = data + row * cpitch;
+  if (k == 0) {
+    const float a = d[0].x, b = d[M].x;
+    d[0] = make_float2(a + b, a - b);
+    continue;
+  }
+  const int kk = M - k;
+  const float2 xk = d[k], xm = d[kk];
+  auto one = [&](float2 a, float2 b, int kw) {
+    const float2 e = make_float2(a.x + b.x, a.y - b.y);
+    const float2 w = cconj(twN[kw]);
+    const float2 o = cmul(make_float2(a.x - b.x, a.y + b.y), w);
+    return make_float2(e.x - o.y, e.y + o.x);
+  };
+  const float2 zk = one(xk, xm, k);
+  if (kk != k) d[kk] = one(xm, xk, kk);
+  d[k] = zk;
+  }
+}
+
+// The elementwise generic kernels loop grid-stride over at most this many
+// workgroups: a 2048^3 mesh has more half-complex values than a launch may
+// have threads (2^32).
+constexpr int64_t kGenMaxBlocks = 1 << 20;
+inline unsigned gen_blocks(int64_t items) {
+  return (unsigned)std::min<int64_t>((items + 255) / 256, kGenMaxBlocks);
+}
+
+// dst = i f2_axis(k) src over the half-complex grid (green_mul); for
+// axis < 0: dst = -sum_a i f2_a src_a (the adjoint sum) of src0..2.
+__global__ void k_gen_green(const float2* __restrict__ s0, const float2* __restrict__ s1,
+                            const float2* __restrict__ s2, float2* __restrict__ dst, int n,
+                            int cpitch, int axis, float f1, int64_t bstride) {
+  const int nh = n / 2 + 1;
+  const int64_t total = (int64_t)n * n * nh;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+  const int iz = (int)(q % nh);
+  const int64_t rxy = q / nh;
+  const int iy = (int)(rxy % n), ix = (int)(rxy / n);
+  const int64_t at = (int64_t)blockIdx.y * bstride + rxy * cpitch + iz;
+  if (axis >= 0) {
+    dst[at] = green_mul(s0[at], ix, iy, iz, GreenArg{axis, n, f1});
+  } else {
+    const float2 a = green_mul(s0[at], ix, iy, iz, GreenArg{0, n, f1});
+    const float2 b = green_mul(s1[at], ix, iy, iz, GreenArg{1, n, f1});
+    const float2 c = green_mul(s2[at], ix, iy, iz, GreenArg{2, n, f1});
+    dst[at] = make_float2(-(a.x + b.x + c.x), -(a.y + b.y + c.y));
+  }
+  }
+}
+
+bool gen_factors_ok(int m) {
+  if (m < 1) return false;
+  for (int p : {2, 3, 5, 7})
+    while (m % p == 0) m /= p;
+  return m == 1;
+}
+
+GenPlan gen_plan(int N) {
+  GenPlan pl{};
+  int m = N;
+  for (int r : {8, 4, 2, 3, 5, 7}) {
+    while (m % r == 0) {
+      if (pl.nst >= kGenMaxStages) fail(FS2_ERR_VALUE, "FFT plan of %d too long", N);
+      pl.R[pl.nst++] = r;
+      m /= r;
+    }
+  }
+  if (m != 1) fail(FS2_ERR_VALUE, "FFT size %d has a prime factor above 7", N);
+  return pl;
+}
+
+struct Gen {
+  static bool supported(int n) { return n >= 4 && n <= 4096 && n % 2 == 0 && gen_factors_ok(n); }
+  static int64_t bstride(int n) { return (int64_t)grid_floats(n) / 2; }
+  // complex transforms of length N along lines (o < nouter, c < ncols)
+  static void lines(const char* name, float2* data, int N, int64_t elem, int nouter,
+                    int64_t ostride, int ncols, int64_t cstride, int nb, bool inv, int n) {
+    const int P = std::max(1, std::min(16, kGenValues / N));
+    const int nchunks = (ncols + P - 1) / P;
+    const double bytes = 16.0 * (double)N * nouter * ncols * nb;
+    launch(name, bytes, k_gen_lines, dim3((unsigned)((int64_t)nouter * nchunks), nb),
+           dim3(kGenThreads), (size_t)2 * P * N * sizeof(float2), data, N, P, elem, ostride,
+           cstride, ncols, nchunks, bstride(n), twiddles(N), gen_plan(N), inv ? 1 : 0);
+  }
+  static void y_pass(float2* c, int n, int nb, bool inv) {
+    const int64_t cp = grid_cpitch(n);
+    lines(inv ? "fft_y_inv" : "fft_y_fwd", c, n, cp, n, (int64_t)n * cp, n / 2 + 1, 1, nb, inv, n);
+  }
+  static void x_pass(float2* c, int n, int nb, bool inv) {
+    const int64_t cp = grid_cpitch(n);
+    lines(inv ? "fft_x_inv" : "fft_x_fwd", c, n, (int64_t)n * cp, n, cp, n / 2 + 1, 1, nb, inv,
+          n);
+  }
+  static void z_r2c(float* grid, int n, int nb) {
+    const int M = n / 2;
+    const int64_t cp = grid_cpitch(n);
+    float2* c = (float2*)grid;
+    lines("fft_z_r2c", c, M, 1, n * nb, (int64_t)n * cp, n, cp, 1, false, n);
+    const int64_t nrows = (int64_t)n * n * nb, items = nrows * (M / 2 + 1);
+    launch("fft_z_split", 16.0 * (double)nrows * (M + 1), k_gen_split, dim3(gen_blocks(items)),
+           dim3(256), 0, c, nrows, M, (int)cp, twiddles(n));
+  }
+  static void z_c2r(float* grid, int n, int nb) {
+    const int M = n / 2;
+    const int64_t cp = grid_cpitch(n);
+    float2* c = (float2*)grid;
+    const int64_t nrows = (int64_t)n * n * nb, items = nrows * (M / 2 + 1);
+    launch("fft_z_merge", 16.0 * (double)nrows * (M + 1), k_gen_merge, dim3(gen_blocks(items)),
+           dim3(256), 0, c, nrows, M, (int)cp, twiddles(n));
+    lines("fft_z_c2r", c, M, 1, n * nb, (int64_t)n * cp, n, cp, 1, true, n);
+  }
+  static void r2c(float* grid, int n, int nb) {
+    z_r2c(grid, n, nb);
+    y_pass((float2*)grid, n, nb, false);
+    x_pass((float2*)grid, n, nb, false);
+  }
+  static void c2r(float* grid, int n, int nb) {
+    x_pass((float2*)grid, n, nb, true);
+    y_pass((float2*)grid, n, nb, true);
+    z_c2r(grid, n, nb);
+  }
+  static void green(const float* s0, const float* s1, const float* s2, float* dst, int n,
+                    int axis, float f1, int nb) {
+    const int64_t total = (int64_t)n * n * (n / 2 + 1);
+    launch("fft_green", 16.0 * (double)total * nb, k_gen_green, dim3(gen_blocks(total), nb),
+           dim3(256), 0, (const float2*)s0,
+           (const float2*)s1, (const float2*)s2, (float2*)dst, n, grid_cpitch(n), axis, f1,
+           bstride(n));
+  }
+  static void force_meshes(float* mesh, float* const out[3], int n, float f1, int nb) {
+    r2c(mesh, n, nb);
+    for (int a = 0; a < 3; ++a) {
+      green(mesh, nullptr, nullptr, out[a], n, a, f1, nb);
+      c2r(out[a], n, nb);
+    }
+  }
+  static void adjoint_green(float* const p[3], int n, float f1) {
+    for (int a = 0; a < 3; ++a) r2c(p[a], n, 1);
+    green(p[0], p[1], p[2], p[0], n, -1, f1, 1);
+    c2r(p[0], n, 1);
+  }
+  static void green_c2r(const float* dk, float* dst, int n, int axis, float f1) {
+    green(dk, nullptr, nullptr, dst, n, axis, f1, 1);
+    c2r(dst, n, 1);
+  }
+};
+
+bool tuned(int n) {
   switch (n) {
     case 4: case 8: case 16: case 32: case 48: case 64: case 96: case 128: case 256:
     case 512: case 1024:
@@ -972,31 +1254,50 @@ bool fft_supported(int n) {
       return false;
   }
 }
+}  // namespace
+
+static_assert(1024LL * 1024 * (((1024 / 2 + 1) + 15) & ~15) < (1LL << 31),
+              "32-bit float2 offsets in the pencil kernels");
+
+bool fft_supported(int n) { return tuned(n) || Gen::supported(n); }
 
 void fft_prepare(int n) {
-  dispatch(n, [&](auto impl) {
-    twiddles(decltype(impl)::N);
-    twiddles(decltype(impl)::M);
-  });
+  if (tuned(n)) {
+    dispatch(n, [&](auto impl) {
+      twiddles(decltype(impl)::N);
+      twiddles(decltype(impl)::M);
+    });
+  } else if (Gen::supported(n)) {
+    twiddles(n);
+    twiddles(n / 2);
+  } else {
+    fail(FS2_ERR_VALUE, "FFT size %d not supported: n must be even with prime factors 2, 3, 5, 7 "
+         "(4 <= n <= 4096)", n);
+  }
 }
 
 void fft_r2c_3d(float* grid, int n, int nb) {
+  if (!tuned(n) && Gen::supported(n)) return Gen::r2c(grid, n, nb);
   dispatch(n, [&](auto impl) { decltype(impl)::r2c(grid, nb); });
 }
 
 void fft_c2r_3d(float* grid, int n, int nb) {
+  if (!tuned(n) && Gen::supported(n)) return Gen::c2r(grid, n, nb);
   dispatch(n, [&](auto impl) { decltype(impl)::c2r(grid, nb); });
 }
 
 void fft_force_meshes(float* mesh, float* const out[3], int n, float f1, int nb) {
+  if (!tuned(n) && Gen::supported(n)) return Gen::force_meshes(mesh, out, n, f1, nb);
   dispatch(n, [&](auto impl) { decltype(impl)::force_meshes(mesh, out, f1, nb); });
 }
 
 void fft_adjoint_green(float* const p[3], int n, float f1) {
+  if (!tuned(n) && Gen::supported(n)) return Gen::adjoint_green(p, n, f1);
   dispatch(n, [&](auto impl) { decltype(impl)::adjoint_green(p, f1); });
 }
 
 void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f1) {
+  if (!tuned(n) && Gen::supported(n)) return Gen::green_c2r(delta_k, dst, n, axis, f1);
   dispatch(n, [&](auto impl) { decltype(impl)::green_c2r(delta_k, dst, axis, f1); });
 }
 

@@ -28,8 +28,8 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
   }
 }
 
-// Call c fills cells 4c..4c+3 (linear index (ix*n + iy)*n + iz); n % 4 == 0,
-// so the four cells sit in one row.
+// Call c fills cells 4c..4c+3 (linear index (ix*n + iy)*n + iz); n is even,
+// so n^3 is a multiple of 4 and every cell gets exactly one value.
 __global__ void k_white_noise(float* __restrict__ g, int n, int ncz, int64_t ncall, uint32_t s0,
                               uint32_t s1) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -43,14 +43,19 @@ __global__ void k_white_noise(float* __restrict__ g, int n, int ncz, int64_t nca
   double sa, ca, sb, cb;
   sincospi(2.0 * u1, &sa, &ca);
   sincospi(2.0 * u3, &sb, &cb);
+  const float v[4] = {(float)(ra * ca), (float)(ra * sa), (float)(rb * cb), (float)(rb * sb)};
+  // the four values are cells 4c..4c+3; with n % 4 == 2 they may span two rows
   const int64_t i = 4 * c;
-  const int64_t row = i / n;
-  const int iz = (int)(i - row * n);
-  float* p = g + row * ncz + iz;
-  p[0] = (float)(ra * ca);
-  p[1] = (float)(ra * sa);
-  p[2] = (float)(rb * cb);
-  p[3] = (float)(rb * sb);
+  int64_t row = i / n;
+  int iz = (int)(i - row * n);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    g[row * ncz + iz] = v[j];
+    if (++iz == n) {
+      iz = 0;
+      ++row;
+    }
+  }
 }
 
 struct Mode {
@@ -194,7 +199,7 @@ __global__ void k_lattice(LatticeArgs o, const float* __restrict__ s0, const flo
 }  // namespace
 
 void white_noise(float* grid, int n, uint64_t seed) {
-  if (n % 4 != 0) fail(FS2_ERR_VALUE, "white noise needs nc divisible by 4 (nc = %d)", n);
+  if (n % 2 != 0) fail(FS2_ERR_VALUE, "white noise needs an even nc (nc = %d)", n);
   const int64_t ncall = (int64_t)n * n * n / 4;
   launch("white_noise", (double)n * n * n * 4.0, k_white_noise, dim3(blocks_for(ncall)),
          dim3(kBlock), 0, grid, n, grid_rpitch(n), ncall, (uint32_t)(seed & 0xffffffffu),
