@@ -25,7 +25,6 @@
 // CIC indices and weights follow pm.cpp:85-133 in FP32 with contraction
 // disabled, exactly as the per-particle atomic kernel does.
 #include <algorithm>
-#include <cstdlib>
 #include <cmath>
 #include <type_traits>
 
@@ -589,186 +588,6 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
   });
 }
 
-// The source ranges of tile_source_ranges in two halves, so that a
-// persistent workgroup can issue the offsets loads of its next tile, work on
-// the current one, and only then scan them: range_issue loads (wavefront 0,
-// one (d, class) pair per lane), range_finish scans and compacts into rb,
-// rpre and *nranges (no barrier; the caller's next barrier publishes them).
-struct RangeLoad {
-  unsigned b, e;
-  bool active;
-};
-__device__ __forceinline__ RangeLoad range_issue(const unsigned* __restrict__ offsets,
-                                                 const TileGeom& g, int Tx, int Ty, int Tz) {
-  const int lane = threadIdx.x & 63;
-  const int d = lane >> 3, cl = lane & 7;
-  const int single = (g.ntx == 1 ? 4 : 0) | (g.nty == 1 ? 2 : 0) | (g.ntz == 1 ? 1 : 0);
-  auto sel = [d](int c) { return (c & d) == d; };
-  RangeLoad r{0u, 0u, false};
-  if ((d & single) == 0 && sel(cl) && (cl == 0 || !sel(cl - 1))) {
-    int end = cl;
-    while (end < 7 && sel(end + 1)) ++end;
-    const unsigned Sx = (d & 4) ? (Tx ? Tx - 1 : g.ntx - 1) : Tx;
-    const unsigned Sy = (d & 2) ? (Ty ? Ty - 1 : g.nty - 1) : Ty;
-    const unsigned Sz = (d & 1) ? (Tz ? Tz - 1 : g.ntz - 1) : Tz;
-    const unsigned S8 = ((Sx * (unsigned)g.nty + Sy) * (unsigned)g.ntz + Sz) * 8u;
-    r.b = offsets[S8 + cl];
-    r.e = offsets[S8 + end + 1];
-    r.active = true;
-  }
-  return r;
-}
-__device__ __forceinline__ void range_finish(const RangeLoad& rl, unsigned* rb, unsigned* rpre,
-                                             int* nranges) {
-  const int lane = threadIdx.x & 63;
-  const unsigned len = rl.active && rl.e > rl.b ? rl.e - rl.b : 0u;
-  unsigned incl = len;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned y = __shfl_up(incl, off);
-    if (lane >= off) incl += y;
-  }
-  const uint64_t nonempty = __ballot(len > 0);
-  const int k = __popcll(nonempty & ((1ull << lane) - 1ull));
-  if (len > 0) {
-    rb[k] = rl.b;
-    rpre[k] = incl - len;
-  }
-  if (lane == 63) {
-    *nranges = __popcll(nonempty);
-    rpre[__popcll(nonempty)] = incl;
-  }
-}
-
-// K4, persistent and pipelined: the same sums as k_tile_paint, bit for bit.
-// Workgroup b runs on XCD b mod 8 and takes every (gridDim/8)-th tile of
-// that XCD's contiguous eighth of the tiles, so the tiles in flight on one
-// XCD are z-neighbours sharing halo lines in its L2.  Per tile: wavefront 0
-// issues the next tile's offsets loads, every thread deposits the records
-// it prefetched (kPrefetch per thread; a cluster's surplus is loaded and
-// deposited in a second loop), wavefront 0 scans the next tile's ranges;
-// barrier; the next tile's record loads are issued and the current tile is
-// written back (and its accumulator cleared) while they are in flight.
-constexpr int kPrefetch = 2;
-template <typename ACC>
-__global__ void __launch_bounds__(kTileBlock)
-k_tile_paint_pipe(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
-                  TileGeom g, float fac, float fix, double fix_inv, float* __restrict__ grid,
-                  unsigned ntot) {
-  extern __shared__ unsigned char smem[];
-  ACC* acc = (ACC*)smem;
-  const int tcells = g.tx * g.ty * g.tz;
-  __shared__ unsigned rb[2][64], rpre[2][65];
-  __shared__ int nranges[2];
-  const unsigned per_xcd = gridDim.x >> 3;
-  const unsigned chunk = (ntot + 7u) >> 3;
-  const unsigned t_end = min(ntot, ((blockIdx.x & 7u) + 1u) * chunk);
-  unsigned t = (blockIdx.x & 7u) * chunk + (blockIdx.x >> 3);
-  if (t >= t_end) return;   // uniform over the workgroup
-  lds_zero(smem, tcells * (int)sizeof(ACC));
-  const float f0 = sizeof(ACC) == 4 ? fac : fac * fix;
-
-  TileCoord tc = tile_coord(g, t);
-  if (threadIdx.x < 64) {
-    const RangeLoad rl = range_issue(offsets + (int64_t)tc.box * g.bins_box, g, tc.Tx, tc.Ty,
-                                     tc.Tz);
-    range_finish(rl, rb[0], rpre[0], &nranges[0]);
-  }
-  __syncthreads();
-  float4 q[kPrefetch];
-  auto prefetch = [&](int buf) {
-    const unsigned total = rpre[buf][nranges[buf]];
-    int r = 0;
-#pragma unroll
-    for (int u = 0; u < kPrefetch; ++u) {
-      const unsigned v = threadIdx.x + u * blockDim.x;
-      if (v < total) {
-        while (v >= rpre[buf][r + 1]) ++r;
-        q[u] = rec[rb[buf][r] + (v - rpre[buf][r])];
-      }
-    }
-  };
-  prefetch(0);
-  for (int k = 0;; ++k) {
-    const int cur = k & 1, nxt = cur ^ 1;
-    const unsigned tn = t + per_xcd;
-    const bool more = tn < t_end;
-    TileCoord tcn = tc;
-    RangeLoad rl{0u, 0u, false};
-    if (more) {
-      tcn = tile_coord(g, tn);
-      if (threadIdx.x < 64)
-        rl = range_issue(offsets + (int64_t)tcn.box * g.bins_box, g, tcn.Tx, tcn.Ty, tcn.Tz);
-    }
-    const int x0 = tc.Tx * g.tx, y0 = tc.Ty * g.ty, z0 = tc.Tz * g.tz;
-    auto deposit = [&](const float4 r4) {
-      const Cic1 cx = cic1(r4.x, g.dx_inv, g.n);
-      const Cic1 cy = cic1(r4.y, g.dx_inv, g.n);
-      const Cic1 cz = cic1(r4.z, g.dx_inv, g.n);
-      const int lx[2] = {cx.i0 - x0, cx.i1 - x0};
-      const int ly[2] = {cy.i0 - y0, cy.i1 - y0};
-      const int lz[2] = {cz.i0 - z0, cz.i1 - z0};
-      const float wx[2] = {cx.w0, cx.w1};
-      const float wy[2] = {cy.w0, cy.w1};
-      const float wz[2] = {cz.w0, cz.w1};
-#pragma unroll
-      for (int ex = 0; ex < 2; ++ex) {
-        if ((unsigned)lx[ex] >= (unsigned)g.tx) continue;
-#pragma unroll
-        for (int ey = 0; ey < 2; ++ey) {
-          if ((unsigned)ly[ey] >= (unsigned)g.ty) continue;
-#pragma unroll
-          for (int ez = 0; ez < 2; ++ez) {
-            if ((unsigned)lz[ez] >= (unsigned)g.tz) continue;
-            // fac*wx*wy*wz, left to right (pm.cpp:122-132)
-            const float w = f0 * wx[ex] * wy[ey] * wz[ez];
-            const int li = (((lx[ex] << g.ly) + ly[ey]) << g.lz) + lz[ez];
-            if constexpr (sizeof(ACC) == 4) {
-              atomicAdd(&acc[li], w);
-            } else {
-              atomicAdd(&acc[li], (ACC)(long long)__float2int_rn(w));
-            }
-          }
-        }
-      }
-    };
-    const unsigned total = rpre[cur][nranges[cur]];
-#pragma unroll
-    for (int u = 0; u < kPrefetch; ++u)
-      if (threadIdx.x + u * blockDim.x < total) deposit(q[u]);
-    {
-      int r = 0;
-      for (unsigned v = threadIdx.x + kPrefetch * blockDim.x; v < total; v += blockDim.x) {
-        while (v >= rpre[cur][r + 1]) ++r;
-        deposit(rec[rb[cur][r] + (v - rpre[cur][r])]);
-      }
-    }
-    if (more && threadIdx.x < 64) range_finish(rl, rb[nxt], rpre[nxt], &nranges[nxt]);
-    __syncthreads();
-    if (more) prefetch(nxt);
-    float* gt = grid + (int64_t)tc.box * g.mstride + ((int64_t)x0 * g.n + y0) * g.ncz + z0;
-    for_tile_groups(g, [&](int c, int off, auto vc) {
-      constexpr int V = decltype(vc)::value;
-      float o[V];
-#pragma unroll
-      for (int j = 0; j < V; ++j) {
-        if constexpr (sizeof(ACC) == 4) {
-          o[j] = acc[c + j] - 1.0f;
-          acc[c + j] = 0.0f;
-        } else {
-          o[j] = (float)((double)(long long)acc[c + j] * fix_inv) - 1.0f;
-          acc[c + j] = 0ull;
-        }
-      }
-      store_group<V>(gt + off, o);
-    });
-    if (!more) break;
-    __syncthreads();
-    t = tn;
-    tc = tcn;
-  }
-}
-
 // Copies the (tx+1) x (ty+1) x (tz+1) halo block of NM meshes starting at
 // cell (x0, y0, z0) (periodic) into LDS, hs[a * hcells + r * hz + lz] with
 // row r = lx * hy + ly.  Mesh rows are tabulated once in LDS (rowoff, one
@@ -874,42 +693,6 @@ k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offs
            h[c110] * cx.w1 * cy.w1 * cz.w0 + h[c110 + 1] * cx.w1 * cy.w1 * cz.w1;
       f[i + a * fstride] = s;
     }
-  }
-}
-
-// CIC readout in bin order without LDS staging: thread j interpolates the
-// three force meshes at record j of the bin-ordered records (tiles in key
-// order, so a wavefront's corners fall in one tile and its halo, and the
-// gathers hit lines the wavefront or its z-neighbours already brought into
-// L2) and writes the forces to the particle's id-order slots.  Same FP32
-// expression order as k_tile_readout and k_readout3.
-__global__ void __launch_bounds__(kBlock)
-k_rec_readout(const float4* __restrict__ rec, int64_t nrec, TileGeom g,
-              const float* __restrict__ m0, const float* __restrict__ m1,
-              const float* __restrict__ m2, float* __restrict__ f, int64_t fstride) {
-  const int64_t j = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (j >= nrec) return;
-  const float4 r4 = rec[j];
-  const unsigned i = __float_as_uint(r4.w);
-  const int64_t moff = g.nbox > 1 ? (int64_t)(i / g.npb) * g.mstride : 0;
-  const Cic1 cx = cic1(r4.x, g.dx_inv, g.n);
-  const Cic1 cy = cic1(r4.y, g.dx_inv, g.n);
-  const Cic1 cz = cic1(r4.z, g.dx_inv, g.n);
-  const int64_t r00 = moff + ((int64_t)cx.i0 * g.n + cy.i0) * g.ncz;
-  const int64_t r01 = moff + ((int64_t)cx.i0 * g.n + cy.i1) * g.ncz;
-  const int64_t r10 = moff + ((int64_t)cx.i1 * g.n + cy.i0) * g.ncz;
-  const int64_t r11 = moff + ((int64_t)cx.i1 * g.n + cy.i1) * g.ncz;
-  const float* ms[3] = {m0, m1, m2};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float* h = ms[a];
-    // pm.cpp:203-218: f = 0; f += (ix0 terms); f += (ix1 terms)
-    float s = 0.f;
-    s += h[r00 + cz.i0] * cx.w0 * cy.w0 * cz.w0 + h[r00 + cz.i1] * cx.w0 * cy.w0 * cz.w1 +
-         h[r01 + cz.i0] * cx.w0 * cy.w1 * cz.w0 + h[r01 + cz.i1] * cx.w0 * cy.w1 * cz.w1;
-    s += h[r10 + cz.i0] * cx.w1 * cy.w0 * cz.w0 + h[r10 + cz.i1] * cx.w1 * cy.w0 * cz.w1 +
-         h[r11 + cz.i0] * cx.w1 * cy.w1 * cz.w0 + h[r11 + cz.i1] * cx.w1 * cy.w1 * cz.w1;
-    f[i + a * fstride] = s;
   }
 }
 
@@ -1216,19 +999,6 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
   const int k = std::min(60, 30 - e);
   const float fix = (float)std::ldexp(1.0, k);
   const double fix_inv = std::ldexp(1.0, -k);
-  if (std::getenv("FS2_PAINT_PIPE")) {   // EXPERIMENT (not for commit)
-    const int per_cu = std::atoi(std::getenv("FS2_PAINT_PIPE"));
-    const unsigned grid_pp = (unsigned)std::max(8, device_cus() * std::max(1, per_cu)) & ~7u;
-    if (deterministic)
-      launch("paint_tile_det", bytes, k_tile_paint_pipe<unsigned long long>, dim3(grid_pp),
-             dim3(kTileBlock), tcells * sizeof(unsigned long long), (const float4*)s.rec,
-             (const unsigned*)s.offsets, g, fac, fix, fix_inv, grid, (unsigned)nblk);
-    else
-      launch("paint_tile", bytes, k_tile_paint_pipe<float>, dim3(grid_pp), dim3(kTileBlock),
-             tcells * sizeof(float), (const float4*)s.rec, (const unsigned*)s.offsets, g, fac,
-             fix, fix_inv, grid, (unsigned)nblk);
-    return;
-  }
   if (deterministic) {
     launch("paint_tile_det", bytes, k_tile_paint<unsigned long long>, dim3((unsigned)nblk),
            dim3(kTileBlock), tcells * sizeof(unsigned long long), (const float4*)s.rec,
@@ -1248,11 +1018,6 @@ void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, 
   const size_t lds = halo_lds(g, 3);
   check_lds(lds, "readout");
   const double bytes = (double)np * 24.0 + 3.0 * 4.0 * (double)n * n * n * g.nbox;
-  if (std::getenv("FS2_READOUT_REC")) {   // EXPERIMENT (not for commit)
-    launch("readout_rec", bytes, k_rec_readout, dim3((unsigned)blocks_for(np)), dim3(kBlock), 0,
-           (const float4*)s.rec, np, g, g0, g1, g2, f, fstride);
-    return;
-  }
   launch("readout_tile", bytes, k_tile_readout, dim3((unsigned)nblk), dim3(kReadoutBlock), lds,
          (const float4*)s.rec, (const unsigned*)s.offsets, g, g0, g1, g2, f, fstride);
 }
