@@ -15,6 +15,7 @@
 // tile paint + tiled readout, HIP events) and the fastest is kept.  The
 // deterministic paint sums are integers, so the choice never changes a bit
 // of the results.
+#include <array>
 #include <cmath>
 #include <vector>
 
@@ -189,10 +190,12 @@ int fs2_batch_create(int nclass, const int* nc_lpt, const int* nc_pm, const int*
     try {
       for (int k = 0; k < nclass; ++k) {
         const int nl = nc_lpt[k], nm = nc_pm[k];
-        if (nl < 4 || nl % 4 != 0 || !fft_supported(nl))
+        if (nl < 4 || nl % 2 != 0 || !fft_supported(nl))
           fail(FS2_ERR_VALUE, "class %d: nc_lpt = %d not supported for 2LPT", k, nl);
         if (nm < 4 || !fft_supported(nm))
           fail(FS2_ERR_VALUE, "class %d: nc_pm = %d is not a supported FFT size", k, nm);
+        fft_prepare(nl);   // twiddle tables now, never inside a graph capture
+        fft_prepare(nm);
         if (nbox[k] < 1) fail(FS2_ERR_VALUE, "class %d: nbox must be >= 1", k);
         if (!(boxsize[k] > 0)) fail(FS2_ERR_VALUE, "class %d: boxsize must be positive", k);
         BatchClass c;
@@ -326,8 +329,17 @@ int fs2_batch_autotune(fs2_batch* b, int reps, double* best_ms) {
         const PosArgs pos{c.field(0), c.np, c.np};
         double best = 1e300;
         TileOpts best_o = c.tile;
-        for (const auto& t : kShapes) {
-          if (!shape_ok(c.nm, t)) continue;
+        // meshes that none of the listed shapes divides (e.g. 20^3, 30^3)
+        // time their current tile alone
+        std::vector<std::array<int, 3>> shapes;
+        for (const auto& t : kShapes)
+          if (shape_ok(c.nm, t)) shapes.push_back({t[0], t[1], t[2]});
+        if (shapes.empty()) {
+          int ext[3];
+          tile_extents(c.nm, c.tile, ext);
+          shapes.push_back({ext[0], ext[1], ext[2]});
+        }
+        for (const auto& t : shapes) {
           TileOpts o = c.tile;
           o.tx = t[0];
           o.ty = t[1];

@@ -44,6 +44,8 @@ CASES = [
     (8, 8, 128.0, 2),
     (8, 16, 128.0, 0),
     (8, 16, 32.0, 3),
+    # generic mixed-radix transforms (12 = 4 x 3, 24 = 8 x 3 outside the tuned sizes)
+    (12, 24, 192.0, 3),
 ]
 
 

@@ -64,6 +64,34 @@ def test_batch_matches_single_box_runs_bitwise(fs, ps):
             k += 1
 
 
+def test_generic_mesh_classes_match_single_box_runs_bitwise(fs, ps):
+    """Classes on meshes outside the tuned transforms (20^3, 60^3: the
+    runtime-planned mixed-radix passes, tiles that no listed shape fits)
+    through the autotuner and a graph replay, box by box against the
+    single-box loop."""
+    from fs2_amd.batch import Ensemble
+    _announce("batch generic mesh sizes vs single-box runs")
+    classes = [(10, 20, 2, 20.0), (30, 60, 1, 60.0)]
+    a0, a1, nstep = 0.1, 1.0, 3
+    ens = Ensemble(classes)
+    seeds = list(range(301, 301 + ens.nbox_total))
+    ens.init_lpt(ps, a0, seeds)
+    for (tile, ms), (nl, nm, nb, L) in zip(ens.autotune(reps=1), classes):
+        assert all(nm % t == 0 for t in tile) and 0 < ms < 1e3
+    ens.set_schedule(a0, a1, nstep)
+    ens.run(graph=True)
+    ens.synchronize()
+    k = 0
+    for ci, (nl, nm, nb, L) in enumerate(classes):
+        for box in range(nb):
+            p = _single_box(fs, ps, nl, nm, L, seeds[k], a0, a1, nstep)
+            for name in ("x", "v"):
+                got = ens.field(ci, box, name)
+                ref = getattr(p, name)
+                assert np.array_equal(got, ref), (ci, box, name, np.abs(got - ref).max())
+            k += 1
+
+
 def test_graph_replays_equal_eager_run(fs, ps):
     """Each replay starts from restored ICs, with eager work and host syncs
     between replays; every replay must reproduce the eager result."""
@@ -111,7 +139,7 @@ def test_batch_errors(fs, ps):
     from fs2_amd.batch import Ensemble
     _announce("batch errors")
     with pytest.raises(RuntimeError):
-        Ensemble([(8, 20, 1, 10.0)])          # 20 is not an FFT size
+        Ensemble([(8, 22, 1, 10.0)])          # 22 = 2 x 11 is not an FFT size
     ens = Ensemble([(8, 16, 2, 16.0)])
     with pytest.raises(RuntimeError):
         ens.run()                             # no schedule
