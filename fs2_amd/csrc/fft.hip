@@ -24,6 +24,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "fs2_internal.h"
 #include "fs2_device.h"
@@ -174,8 +175,26 @@ __device__ __forceinline__ int lds_elem(int a0, int j, int r) {
   }
 }
 
+// Barriers between the LDS exchanges of a transform.  Pencils spread over
+// several wavefronts need the workgroup barrier.  A row whose threads all sit
+// in one wavefront (z passes with T <= 64 threads per row, rows packed 64/T
+// to a wavefront, each in its own LDS slice) needs only the wavefront's own
+// LDS instructions kept in program order, which the LDS executes in order:
+// the waves of a workgroup then run their rows without waiting for each
+// other.
+struct BlockSync {
+  static __device__ __forceinline__ void sync() { __syncthreads(); }
+};
+struct WaveSync {
+  static __device__ __forceinline__ void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
 // One Stockham stage over one pencil held in LDS at s[L::at(j)].
-template <int N, int T, class L, int Ns, int R>
+template <int N, int T, class L, int Ns, int R, class S = BlockSync>
 __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* __restrict__ tw) {
   static_assert(N % (R * T) == 0, "plan: stage does not divide evenly");
   constexpr int B = N / (R * T);
@@ -187,7 +206,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
 #pragma unroll
     for (int r = 0; r < R; ++r) v[b][r] = s[lds_elem<L, N / R>(a0, j, r)];
   }
-  __syncthreads();
+  S::sync();
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const int j = t + b * T;
@@ -202,7 +221,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
 #pragma unroll
     for (int r = 0; r < R; ++r) s[lds_elem<L, Ns>(a0, base, r)] = v[b][r];
   }
-  __syncthreads();
+  S::sync();
 }
 
 // Register-resident variant.  Thread t holds elements t + i T (i < N/T) of
@@ -212,7 +231,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
 // the first stage and the inner stages go through LDS: two LDS passes fewer
 // and three barriers instead of 2S + 1.  Starts with a barrier, so the LDS
 // pencil may still be in use by the caller on entry.
-template <int N, int T, class L, int R>
+template <int N, int T, class L, int R, class S = BlockSync>
 __device__ __forceinline__ void stage_first(const float2* x, float2* s, int t) {
   constexpr int B = N / (R * T);
 #pragma unroll
@@ -225,7 +244,7 @@ __device__ __forceinline__ void stage_first(const float2* x, float2* s, int t) {
 #pragma unroll
     for (int r = 0; r < R; ++r) s[L::at(j * R + r)] = v[r];
   }
-  __syncthreads();
+  S::sync();
 }
 
 template <int N, int T, class L, int Ns, int R>
@@ -248,34 +267,39 @@ __device__ __forceinline__ void stage_last(float2* x, const float2* s, int t,
   }
 }
 
-template <int N, int T, class L, int Ns, int R, int... Rest>
+template <int N, int T, class L, class S, int Ns, int R, int... Rest>
 __device__ __forceinline__ void regs_stages(float2* x, float2* s, int t,
                                             const float2* __restrict__ tw) {
   if constexpr (sizeof...(Rest) == 0) {
     stage_last<N, T, L, Ns, R>(x, s, t, tw);
   } else {
-    stockham_stage<N, T, L, Ns, R>(s, t, tw);
-    regs_stages<N, T, L, Ns * R, Rest...>(x, s, t, tw);
+    stockham_stage<N, T, L, Ns, R, S>(s, t, tw);
+    regs_stages<N, T, L, S, Ns * R, Rest...>(x, s, t, tw);
   }
 }
 
-template <int N, int T, class L, int R1, int... Rest>
+template <int N, int T, class L, class S, int R1, int... Rest>
 __device__ __forceinline__ void run_plan_regs(float2* x, float2* s, int t,
                                               const float2* __restrict__ tw, Radices<R1, Rest...>) {
   if constexpr (sizeof...(Rest) == 0) {
     static_assert(T == 1 && R1 == N, "single-stage plans run in one thread");
     DFT<N>::run(x);
   } else {
-    __syncthreads();
-    stage_first<N, T, L, R1>(x, s, t);
-    regs_stages<N, T, L, R1, Rest...>(x, s, t, tw);
+    S::sync();
+    stage_first<N, T, L, R1, S>(x, s, t);
+    regs_stages<N, T, L, S, R1, Rest...>(x, s, t, tw);
   }
 }
 
-template <int N, class L>
+template <int N, class L, class S = BlockSync>
 __device__ __forceinline__ void fft_regs(float2* x, float2* s, int t, const float2* __restrict__ tw) {
-  run_plan_regs<N, Plan<N>::T, L>(x, s, t, tw, typename Plan<N>::R{});
+  run_plan_regs<N, Plan<N>::T, L, S>(x, s, t, tw, typename Plan<N>::R{});
 }
+
+// The barrier a z-pass row needs: the wavefront's own when the row's T
+// threads sit in one wavefront.
+template <int M>
+using RowSync = std::conditional_t<(64 % Plan<M>::T == 0), WaveSync, BlockSync>;
 
 // Copies the N-entry twiddle table into LDS (read there by every stage; a
 // barrier follows before first use).
@@ -591,11 +615,12 @@ k_fft_z_r2c(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
     const int m = t + i * T;
     x[i] = valid ? r[m] : make_float2(0.f, 0.f);
   }
-  fft_regs<M, L>(x, sp, t, twM);
-  __syncthreads();   // every thread is done reading the last stage's input
+  using S = RowSync<M>;
+  fft_regs<M, L, S>(x, sp, t, twM);
+  S::sync();   // every thread of the row is done reading the last stage's input
 #pragma unroll
   for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = x[i];
-  __syncthreads();
+  S::sync();
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
@@ -641,11 +666,13 @@ k_fft_z_r2c_lt(float2* data, int64_t nrows, int cpitch, const float2* __restrict
     const int m = t + i * T;
     x[i] = valid ? r[m] : make_float2(0.f, 0.f);
   }
-  fft_regs<M, L>(x, sp, t, stM);   // starts with a barrier: the twiddles are staged
-  __syncthreads();   // every thread is done reading the last stage's input
+  using S = RowSync<M>;
+  __syncthreads();   // the twiddles are staged
+  fft_regs<M, L, S>(x, sp, t, stM);
+  S::sync();   // every thread of the row is done reading the last stage's input
 #pragma unroll
   for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = x[i];
-  __syncthreads();
+  S::sync();
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
@@ -679,19 +706,14 @@ k_fft_z_c2r(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
   const bool valid = row < nrows;
   float2* r = data + row * cpitch;
   float2* sp = s + p * LS;
-#pragma unroll
-  for (int i = 0; i < M / T; ++i) {
-    const int m = t + i * T;
-    sp[L::at(m)] = valid ? r[m] : make_float2(0.f, 0.f);
-  }
-  if (t == 0) sp[L::at(M)] = valid ? r[M] : make_float2(0.f, 0.f);
-  __syncthreads();
+  // X[k] and X[M - k] straight from the row (the mirrored loads hit the
+  // lines the direct ones bring in)
   float2 z[M / T];
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
-    float2 A = sp[L::at(k)];
-    float2 B = sp[L::at(M - k)];
+    float2 A = valid ? r[k] : make_float2(0.f, 0.f);
+    float2 B = valid ? r[M - k] : make_float2(0.f, 0.f);
     if (k == 0) {
       A.y = 0.f;
       B.y = 0.f;
@@ -701,7 +723,7 @@ k_fft_z_c2r(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
     const float2 O = cmul(csub(A, Bc), cconj(twN[k]));
     z[i] = cconj(cadd(E, mul_posi(O)));
   }
-  fft_regs<M, L>(z, sp, t, twM);
+  fft_regs<M, L, RowSync<M>>(z, sp, t, twM);
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int m = t + i * T;
@@ -728,6 +750,8 @@ k_fft_z_c2r_p(float2* data, int64_t nrows, int cpitch, const float2* __restrict_
     stN[i] = twN[i];
   }
   const int tid = threadIdx.x;
+  using S = RowSync<M>;
+  __syncthreads();   // the twiddles are staged
   for (int64_t row0 = (int64_t)blockIdx.x * P; row0 < nrows; row0 += (int64_t)gridDim.x * P) {
     // opaque thread coordinates: keeps the compiler from holding every LDS
     // address of the transform live across iterations
@@ -737,17 +761,15 @@ k_fft_z_c2r_p(float2* data, int64_t nrows, int cpitch, const float2* __restrict_
     const int64_t row = row0 + p;
     const bool valid = row < nrows;
     float2* r = data + row * cpitch;
-    __syncthreads();   // staging, and the previous row's LDS reads, are done
-#pragma unroll
-    for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = valid ? r[t + i * T] : make_float2(0.f, 0.f);
-    if (t == 0) sp[L::at(M)] = valid ? r[M] : make_float2(0.f, 0.f);
-    __syncthreads();
+    // X[k] and X[M - k] straight from the row (the mirrored loads hit the
+    // lines the direct ones bring in); the transform's first barrier orders
+    // its LDS writes after the previous row's last reads
     float2 z[M / T];
 #pragma unroll
     for (int i = 0; i < M / T; ++i) {
       const int k = t + i * T;
-      float2 A = sp[L::at(k)];
-      float2 B = sp[L::at(M - k)];
+      float2 A = valid ? r[k] : make_float2(0.f, 0.f);
+      float2 B = valid ? r[M - k] : make_float2(0.f, 0.f);
       if (k == 0) {
         A.y = 0.f;
         B.y = 0.f;
@@ -757,7 +779,7 @@ k_fft_z_c2r_p(float2* data, int64_t nrows, int cpitch, const float2* __restrict_
       const float2 O = cmul(csub(A, Bc), cconj(stN[k]));
       z[i] = cconj(cadd(E, mul_posi(O)));
     }
-    fft_regs<M, L>(z, sp, t, stM);
+    fft_regs<M, L, S>(z, sp, t, stM);
 #pragma unroll
     for (int i = 0; i < M / T; ++i)
       if (valid) r[t + i * T] = cconj(z[i]);
