@@ -59,8 +59,12 @@ struct fs2_pm {
   int variant = FS2_PAINT_SORTED;
   bool mesh_is_k = false;
   fs2::PaintScratch paint;     // tiled-paint scratch, grown on demand
+  fs2::TileOpts tiles;         // paint tiles: column tiles where the fused readout runs
   uint64_t binned_version = 0; // x_version of the particles the scratch bins
   uint64_t binned_np = 0;
+  // force[a] hold the spectra before their z c2r (fused readout); the real
+  // meshes are made on demand
+  bool force_spectral = false;
 };
 
 namespace fs2 {
@@ -782,6 +786,7 @@ int fs2_pm_create(int nc_pm, double pm_factor, double boxsize, fs2_pm** out) {
       delete pm;
       throw;
     }
+    pm->tiles = column_tile_opts(nc_pm);
     pm->density.nc = nc_pm;
     pm->density.d = pm->mesh;
     pm->density.owning = false;
@@ -832,9 +837,9 @@ int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density) {
       fill_grid(pm->mesh, n, -1.0f);
       paint_atomic(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending);
     } else {
-      ensure_paint_scratch(pm->paint, p->np_local, paint_tiled_bins(n));
+      ensure_paint_scratch(pm->paint, p->np_local, paint_tiled_bins(n, pm->tiles));
       paint_tiled(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending,
-                  pm->variant == FS2_PAINT_SORTED, pm->paint);
+                  pm->variant == FS2_PAINT_SORTED, pm->paint, pm->tiles);
       if (p->wrap_pending) touch_x(p);   // the binning kernel wrapped x in place
     }
     pm->binned_version = pm->variant == FS2_PAINT_GLOBAL ? 0 : p->x_version;
@@ -867,22 +872,31 @@ int fs2_pm_compute_force(fs2_pm* pm, fs2_particles* p) {
     const int n = pm->nc;
     // pm.cpp:472: f1 = -1/n^3 / (2 pi / boxsize), boxsize held as a Float
     const float f1 = (float)(-1.0 / std::pow((double)n, 3.0) / (2.0 * M_PI / (double)pm->boxsize_f));
-    fft_force_meshes(pm->mesh, pm->force, n, f1);
-    pm->mesh_is_k = false;
-    pm->status = PM_FORCE_DONE;   // the density mesh no longer holds delta
-    flush_wrap(p);
     // The paint's bin records hold the positions it binned.  They are reused
     // only when x cannot have changed since: not after the device planes were
     // handed out (fs2_particles_device_fields), whose writes bump no version.
-    if (pm->binned_version != 0 && pm->binned_version == p->x_version &&
-        pm->binned_np == p->np_local && !p->planes_exposed) {
-      readout_tiled(pm->force[0], pm->force[1], pm->force[2], p->f(), (int64_t)p->np_alloc,
-                    (int64_t)p->np_local, n, pm->boxsize_f, pm->paint);
+    flush_wrap(p);
+    const bool reuse_bins = pm->binned_version != 0 && pm->binned_version == p->x_version &&
+                            pm->binned_np == p->np_local && !p->planes_exposed;
+    pm->mesh_is_k = false;
+    pm->status = PM_FORCE_DONE;   // the density mesh no longer holds delta
+    if (reuse_bins && column_readout_supported(n)) {
+      // the z c2r of the three force meshes fused into the readout
+      fft_force_spectra(pm->mesh, pm->force, n, f1);
+      readout_columns(pm->force[0], pm->force[1], pm->force[2], p->f(), (int64_t)p->np_alloc,
+                      (int64_t)p->np_local, n, pm->boxsize_f, pm->paint, pm->tiles);
+      pm->force_spectral = true;
     } else {
-      readout3(pos_of(p), pm->force[0], pm->force[1], pm->force[2], p->f(), n, pm->boxsize_f);
+      fft_force_meshes(pm->mesh, pm->force, n, f1);
+      pm->force_spectral = false;
+      if (reuse_bins) {
+        readout_tiled(pm->force[0], pm->force[1], pm->force[2], p->f(), (int64_t)p->np_alloc,
+                      (int64_t)p->np_local, n, pm->boxsize_f, pm->paint, pm->tiles);
+      } else {
+        readout3(pos_of(p), pm->force[0], pm->force[1], pm->force[2], p->f(), n, pm->boxsize_f);
+      }
     }
     p->a_f = p->a_x;
-    pm->status = PM_FORCE_DONE;
   });
 }
 
@@ -896,6 +910,10 @@ int fs2_pm_get_forces(fs2_pm* pm, fs2_particles* p) {
 int fs2_pm_get_force_mesh(fs2_pm* pm, int axis, float* out) {
   return guard([&] {
     if (axis < 0 || axis > 2) fail(FS2_ERR_VALUE, "axis %d", axis);
+    if (pm->force_spectral) {   // finish the spectra the fused readout left
+      fft_force_finish(pm->force, pm->nc);
+      pm->force_spectral = false;
+    }
     fs2_grid view;
     view.nc = pm->nc;
     view.d = pm->force[axis];

@@ -123,6 +123,13 @@ void fft_green_c2r_3d(const float* delta_k, float* dst, int n, int axis, float f
 // gradient spectra, then y and z c2r passes of each force mesh.  The density
 // mesh is left holding its y-transformed intermediate.
 void fft_force_meshes(float* mesh, float* const out[3], int n, float f1, int nb = 1);
+// fft_force_meshes in two parts (tuned sizes): the spectra of the three force
+// meshes after the x and y inverse passes (half-complex [x][y][kz] rows),
+// and the three z c2r passes that turn them into the real meshes.
+void fft_force_spectra(float* mesh, float* const out[3], int n, float f1);
+void fft_force_finish(float* const out[3], int n);
+// The FP32 twiddle table e^{-2 pi i k/N}, k < N, of the transforms.
+const float2* fft_twiddle_table(int N);
 // Transpose of the force-mesh operator: p[0] <- -sum_a M_a p[a] (real meshes,
 // M_a = c2r . (i f1 k_a/|k|^2) . r2c); p[1] and p[2] are clobbered.
 void fft_adjoint_green(float* const p[3], int n, float f1);
@@ -171,6 +178,15 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
 void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, int64_t fstride,
                    int64_t np, int n, float boxsize, const PaintScratch& s,
                    const TileOpts& o = TileOpts());
+// Fused z c2r + readout for meshes where column_readout_supported(n): the
+// paint bins particles into column_tile_opts(n) tiles, and readout_columns
+// finishes the force spectra s[a] (fft_force_spectra) row by row in LDS and
+// reads the forces out of them, without writing the real force meshes.
+bool column_readout_supported(int n);
+TileOpts column_tile_opts(int n);
+void readout_columns(const float* s0, const float* s1, const float* s2, float* f,
+                     int64_t fstride, int64_t np, int n, float boxsize, const PaintScratch& s,
+                     const TileOpts& o);
 // Adjoint kernels over the bins of the last paint_tiled call (positions
 // unchanged since).  paint_values3_tiled: out_a(c) = sum_p W(x_p - c) v_a(p)
 // for the value planes v_a = vals[i + a*vstride], int64 fixed-point sums, every

@@ -338,6 +338,38 @@ def test_tiled_readout_matches_gather_readout(fs, n_p, nc):
     assert rel < 1e-5, rel
 
 
+@pytest.mark.timeout(600)
+def test_column_readout_matches_unfused_path(fs):
+    """At 1024^3 compute_force fuses the z c2r of the three force meshes into
+    the readout (paint bins in 2 x 4 x 512 column tiles, rows transformed in
+    LDS, no real force mesh written).  Forces must equal, bit for bit, those
+    of the unfused path (z c2r passes writing the real meshes, per-particle
+    gather readout), and force_mesh must return the same real meshes either
+    way (finished on demand after the fused path)."""
+    _announce("column readout vs unfused path, 128^3 particles, 1024^3 mesh")
+    n_p, nc, L = 128, 1024, 1024.0
+    x = np.mod(_lattice_positions(n_p, L, 2.0, seed=17), L)
+    # a dense clump too: many particles per column, all three class bits
+    x[:50000] = np.mod(512.0 + 3.0 * np.random.default_rng(18).standard_normal((50000, 3)), L)
+    fs.pm.init(nc, nc / n_p, L)
+    fs.pm.set_paint_variant("sorted")
+    p = fs.Particles(n_p**3, L)
+    p.append(x)
+    p.set_times(0.5, 0.5, 0.0)
+    fs.pm.force(p)                 # bins current -> fused column readout
+    f_col = p.force
+    assert np.isfinite(f_col).all() and np.abs(f_col).max() > 0
+    mesh_col = [fs.pm.force_mesh(a)[::7, 3::11, :].copy() for a in range(3)]
+    p.set_times(0.5, 0.5, -1.0)
+    fs.pm.send_positions(p)
+    fs.pm.compute_density(p)
+    p.set_field("x", p.x)          # same values, new version -> unfused path
+    fs.pm.compute_force(p)
+    assert np.array_equal(p.force, f_col), np.abs(p.force - f_col).max()
+    for a in range(3):
+        assert np.array_equal(fs.pm.force_mesh(a)[::7, 3::11, :], mesh_col[a]), a
+
+
 def test_mem_named_allocation(fs):
     """fs.mem.Mem (py/fs/mem.py): sizes rounded up to 16 bytes (mem.cpp:48-50),
     nothing allocated for size 0, MemoryError when the device cannot hold it."""
