@@ -727,7 +727,8 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
   constexpr int T = Plan<M>::T;
   static_assert(T == 64, "one wavefront per z row");
   using L = ContigLayout;
-  constexpr int LS = L::size(M);   // float2 per row slice: >= M, i.e. one real row
+  constexpr int LS = L::size(M);   // float2 per row slice: > M, a real row and one more float
+  static_assert(LS > M, "row slice holds n + 1 floats");
   __shared__ float2 rows[kColRows * LS];
   __shared__ float2 stM[M];
   __shared__ float2 stN[M];
@@ -772,13 +773,11 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
     }
     fft_regs<M, L, WaveSync>(z, sp, t, stM);
     WaveSync::sync();   // the row's lanes are done reading the last stage's input
-    float* real = (float*)sp;
+    // the real row, x[2m] + i x[2m+1] = conj(z[m]), and x[n] = x[0] after it,
+    // so a stencil's z + 1 corner is always the next float
 #pragma unroll
-    for (int i = 0; i < M / T; ++i) {
-      const int m = t + i * T;
-      real[2 * m] = z[i].x;   // conj(z): x_even + i x_odd
-      real[2 * m + 1] = -z[i].y;
-    }
+    for (int i = 0; i < M / T; ++i) sp[t + i * T] = cconj(z[i]);
+    if (t == 0) ((float*)sp)[2 * M] = z[0].x;
     __syncthreads();
     const float* h = (const float*)rows;
     constexpr int RS = 2 * LS;   // floats per row slice
@@ -789,7 +788,7 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
       const Cic1 cz = cic1(r4.z, g.dx_inv, n);
       const int q00 = ((cx.i0 - x0) * (kColY + 1) + (cy.i0 - y0)) * RS;
       const int q01 = q00 + RS, q10 = q00 + (kColY + 1) * RS, q11 = q10 + RS;
-      const int z0 = cz.i0, z1 = cz.i1;
+      const int z0 = cz.i0, z1 = cz.i0 + 1;
       // pm.cpp:203-218: f = 0; f += (ix0 terms); f += (ix1 terms)
       float v = 0.f;
       v += h[q00 + z0] * cx.w0 * cy.w0 * cz.w0 + h[q00 + z1] * cx.w0 * cy.w0 * cz.w1 +
