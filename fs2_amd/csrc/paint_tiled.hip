@@ -701,20 +701,21 @@ k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offs
 
 // ---- fused z c2r + readout (columns of the 1024^3 mesh) -------------------------------
 // The last three passes of the force transform (z c2r of F_x, F_y, F_z) and
-// the readout in one kernel: a workgroup owns a column of kColX x kColY
-// cells through all z.  For each axis it transforms the (kColX+1) x
-// (kColY+1) z rows its particles' stencils touch (one wavefront per row,
-// the row's LDS slice serving as the transform's scratch and then holding
-// the real row), and reads the force out at the column's particles.  The
-// real force meshes are never written: 1.875x the spectra are read, against
-// a write and a 1.29x halo read of three real meshes plus the spectra.
-// Particles come from the bins of a paint with kColX x kColY x tz tiles
-// (z fastest, so a column's bins are one contiguous range).  The row
+// the readout in one kernel.  A workgroup walks a strip of kColStrip
+// columns of kColX x kColY cells (all z) along y, one axis at a time.  For
+// each column it transforms the z rows its particles' stencils touch,
+// (kColX + 1) x (kColY + 1) of them, one wavefront per row, keeping the
+// kColX + 1 rows it shares with the next column: the row's LDS slice serves
+// as the transform's scratch and then holds the real row.  It then reads
+// the force out at the column's particles.  The real force meshes are never
+// written; each spectrum row is read and transformed ~1.5 times on
+// average.  Particles come from the bins of a paint with kColX x kColY x tz
+// tiles (z fastest, so a column's bins are one contiguous range).  The row
 // transforms are the z c2r pass's arithmetic and the interpolation is
 // k_tile_readout's, so forces are bitwise those of the unfused path.
 constexpr int kColX = 2, kColY = 4;
 constexpr int kColRows = (kColX + 1) * (kColY + 1);
-constexpr int kColGroup = 8;
+constexpr int kColStrip = 8;
 
 template <int M>
 __global__ void __launch_bounds__(kColRows * 64) __attribute__((amdgpu_waves_per_eu(8)))
@@ -729,6 +730,7 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
   using L = ContigLayout;
   constexpr int LS = L::size(M);   // float2 per row slice: > M, a real row and one more float
   static_assert(LS > M, "row slice holds n + 1 floats");
+  constexpr int HY = kColY + 1;    // y rows of a column, a ring of HY slices per x row
   __shared__ float2 rows[kColRows * LS];
   __shared__ float2 stM[M];
   __shared__ float2 stN[M];
@@ -738,66 +740,84 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
   }
   const int n = g.n;
   const int cp = g.ncz / 2;
-  // columns in groups of kColGroup x kColGroup, y fastest: a group's
-  // workgroups run together on one XCD, so the z rows that neighbouring
-  // columns share come from its L2
-  const int ngy = n / kColY / kColGroup;
+  // strips x fastest: the strips next to each other in x share their
+  // boundary rows and run together on one XCD (its L2 serves the second read)
+  const int nsx = n / kColX;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int grp = b / (kColGroup * kColGroup), in = b - grp * (kColGroup * kColGroup);
-  const int Gx = grp / ngy, Gy = grp - Gx * ngy;
-  const int Cx = Gx * kColGroup + in / kColGroup, Cy = Gy * kColGroup + in % kColGroup;
-  const int x0 = Cx * kColX, y0 = Cy * kColY;
-  const unsigned col = (unsigned)(Cx * g.nty + Cy) * (unsigned)g.ntz;
-  const unsigned pb = offsets[col * 8u], pe = offsets[(col + (unsigned)g.ntz) * 8u];
-  // this wavefront's z row: (x0 + hx, y0 + hy), periodic
+  const int Sy = b / nsx, Cx = b - Sy * nsx;
+  const int x0 = Cx * kColX;
   const int w0 = threadIdx.x >> 6, t0 = threadIdx.x & 63;
-  const int hx = w0 / (kColY + 1), hy = w0 - hx * (kColY + 1);
-  int gx = x0 + hx, gy = y0 + hy;
-  if (gx >= n) gx -= n;
-  if (gy >= n) gy -= n;
-  const int64_t roff = ((int64_t)gx * n + gy) * cp;
+  const float* h = (const float*)rows;
+  constexpr int RS = 2 * LS;   // floats per row slice
   __syncthreads();   // the twiddles are staged
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    // opaque copies of the lane and wave: without them the compiler keeps
-    // every LDS address of the transform live across the axis loop
-    int t = t0, w = w0;
-    asm volatile("" : "+v"(t), "+v"(w));
-    float2* sp = rows + w * LS;
-    const float2* r = (a == 0 ? s0 : a == 1 ? s1 : s2) + roff;
-    float2 z[M / T];
+    const float2* sa = a == 0 ? s0 : a == 1 ? s1 : s2;
+#pragma unroll 1
+    for (int c = 0; c < kColStrip; ++c) {
+      const int Cy = Sy * kColStrip + c;
+      const int y0 = Cy * kColY;
+      // this wavefront's row: all HY x (kColX+1) rows for the strip's first
+      // column, then the kColY new ones per x row (the y0 row is the last
+      // column's y0 + kColY row, still in its slice)
+      int hx, hy;
+      bool busy;
+      if (c == 0) {
+        hx = w0 / HY;
+        hy = w0 - hx * HY;
+        busy = true;
+      } else {
+        hx = w0 / kColY;
+        hy = 1 + (w0 - hx * kColY);
+        busy = w0 < (kColX + 1) * kColY;
+      }
+      if (busy) {
+        // opaque copies of the lane and slice: without them the compiler
+        // keeps every LDS address of the transform live across the loops
+        int t = t0, slot = hx * HY + (kColY * c + hy) % HY;
+        asm volatile("" : "+v"(t), "+v"(slot));
+        float2* sp = rows + slot * LS;
+        int gx = x0 + hx, gy = y0 + hy;
+        if (gx >= n) gx -= n;
+        if (gy >= n) gy -= n;
+        const float2* r = sa + ((int64_t)gx * n + gy) * cp;
+        float2 z[M / T];
 #pragma unroll
-    for (int i = 0; i < M / T; ++i) {
-      const int k = t + i * T;
-      z[i] = c2r_merge(r[k], r[M - k], stN[k], k == 0);
-    }
-    fft_regs<M, L, WaveSync>(z, sp, t, stM);
-    WaveSync::sync();   // the row's lanes are done reading the last stage's input
-    // the real row, x[2m] + i x[2m+1] = conj(z[m]), and x[n] = x[0] after it,
-    // so a stencil's z + 1 corner is always the next float
+        for (int i = 0; i < M / T; ++i) {
+          const int k = t + i * T;
+          z[i] = c2r_merge(r[k], r[M - k], stN[k], k == 0);
+        }
+        fft_regs<M, L, WaveSync>(z, sp, t, stM);
+        WaveSync::sync();   // the row's lanes are done reading the last stage's input
+        // the real row, x[2m] + i x[2m+1] = conj(z[m]), and x[n] = x[0]
+        // after it, so a stencil's z + 1 corner is always the next float
 #pragma unroll
-    for (int i = 0; i < M / T; ++i) sp[t + i * T] = cconj(z[i]);
-    if (t == 0) ((float*)sp)[2 * M] = z[0].x;
-    __syncthreads();
-    const float* h = (const float*)rows;
-    constexpr int RS = 2 * LS;   // floats per row slice
-    for (unsigned j = pb + threadIdx.x; j < pe; j += blockDim.x) {
-      const float4 r4 = rec[j];
-      const Cic1 cx = cic1(r4.x, g.dx_inv, n);
-      const Cic1 cy = cic1(r4.y, g.dx_inv, n);
-      const Cic1 cz = cic1(r4.z, g.dx_inv, n);
-      const int q00 = ((cx.i0 - x0) * (kColY + 1) + (cy.i0 - y0)) * RS;
-      const int q01 = q00 + RS, q10 = q00 + (kColY + 1) * RS, q11 = q10 + RS;
-      const int z0 = cz.i0, z1 = cz.i0 + 1;
-      // pm.cpp:203-218: f = 0; f += (ix0 terms); f += (ix1 terms)
-      float v = 0.f;
-      v += h[q00 + z0] * cx.w0 * cy.w0 * cz.w0 + h[q00 + z1] * cx.w0 * cy.w0 * cz.w1 +
-           h[q01 + z0] * cx.w0 * cy.w1 * cz.w0 + h[q01 + z1] * cx.w0 * cy.w1 * cz.w1;
-      v += h[q10 + z0] * cx.w1 * cy.w0 * cz.w0 + h[q10 + z1] * cx.w1 * cy.w0 * cz.w1 +
-           h[q11 + z0] * cx.w1 * cy.w1 * cz.w0 + h[q11 + z1] * cx.w1 * cy.w1 * cz.w1;
-      f[__float_as_uint(r4.w) + a * fstride] = v;
+        for (int i = 0; i < M / T; ++i) sp[t + i * T] = cconj(z[i]);
+        if (t == 0) ((float*)sp)[2 * M] = z[0].x;
+      }
+      __syncthreads();
+      const unsigned col = (unsigned)(Cx * g.nty + Cy) * (unsigned)g.ntz;
+      const unsigned pb = offsets[col * 8u], pe = offsets[(col + (unsigned)g.ntz) * 8u];
+      const int ybase = kColY * c;
+      for (unsigned j = pb + threadIdx.x; j < pe; j += blockDim.x) {
+        const float4 r4 = rec[j];
+        const Cic1 cx = cic1(r4.x, g.dx_inv, n);
+        const Cic1 cy = cic1(r4.y, g.dx_inv, n);
+        const Cic1 cz = cic1(r4.z, g.dx_inv, n);
+        const int lx = cx.i0 - x0, ly = ybase + (cy.i0 - y0);
+        const int q00 = (lx * HY + ly % HY) * RS, q01 = (lx * HY + (ly + 1) % HY) * RS;
+        const int q10 = q00 + HY * RS, q11 = q01 + HY * RS;
+        const int z0 = cz.i0, z1 = cz.i0 + 1;
+        // pm.cpp:203-218: f = 0; f += (ix0 terms); f += (ix1 terms)
+        float v = 0.f;
+        v += h[q00 + z0] * cx.w0 * cy.w0 * cz.w0 + h[q00 + z1] * cx.w0 * cy.w0 * cz.w1 +
+             h[q01 + z0] * cx.w0 * cy.w1 * cz.w0 + h[q01 + z1] * cx.w0 * cy.w1 * cz.w1;
+        v += h[q10 + z0] * cx.w1 * cy.w0 * cz.w0 + h[q10 + z1] * cx.w1 * cy.w0 * cz.w1 +
+             h[q11 + z0] * cx.w1 * cy.w1 * cz.w0 + h[q11 + z1] * cx.w1 * cy.w1 * cz.w1;
+        f[__float_as_uint(r4.w) + a * fstride] = v;
+      }
+      __syncthreads();   // the rows are read before the next column overwrites them
     }
-    __syncthreads();   // the rows are read before the next axis overwrites them
   }
 }
 
@@ -1145,15 +1165,14 @@ void readout_columns(const float* s0, const float* s1, const float* s2, float* f
   if (!column_readout_supported(n)) fail(FS2_ERR_VALUE, "column readout: mesh %d", n);
   const TileGeom g = make_geom(n, boxsize, o, np);
   check_scratch(g, np, s);
-  if (g.tx != kColX || g.ty != kColY || g.nbox != 1 || (n / kColX) % kColGroup ||
-      (n / kColY) % kColGroup)
+  if (g.tx != kColX || g.ty != kColY || g.nbox != 1 || (n / kColY) % kColStrip)
     fail(FS2_ERR_RUNTIME, "column readout needs the bins of %dx%dx* paint tiles (have %dx%dx%d)",
          kColX, kColY, g.tx, g.ty, g.tz);
   constexpr int M = 512;
-  const unsigned ncol = (unsigned)(n / kColX) * (unsigned)(n / kColY);
+  const unsigned nstrip = (unsigned)(n / kColX) * (unsigned)(n / kColY / kColStrip);
   // algorithmic bytes: the three spectra read once, positions and forces
   const double bytes = 3.0 * (double)n * n * (M + 1) * 8.0 + (double)np * 24.0;
-  launch("readout_columns", bytes, k_column_readout<M>, dim3(ncol), dim3(kColRows * 64), 0,
+  launch("readout_columns", bytes, k_column_readout<M>, dim3(nstrip), dim3(kColRows * 64), 0,
          (const float2*)s0, (const float2*)s1, (const float2*)s2, (const float4*)s.rec,
          (const unsigned*)s.offsets, g, fft_twiddle_table(M), fft_twiddle_table(2 * M), f,
          fstride);
