@@ -18,6 +18,7 @@ struct fs2_grid {
   int nc = 0;
   float* d = nullptr;   // padded [nc][nc][2(nc/2+1)]
   bool owning = true;
+  fs2_pm* owner = nullptr;   // the PM whose density mesh this view shows
 };
 
 struct fs2_particles {
@@ -65,6 +66,10 @@ struct fs2_pm {
   // force[a] hold the spectra before their z c2r (fused readout); the real
   // meshes are made on demand
   bool force_spectral = false;
+  // compute_density binned the particles but left the tile kernel for
+  // compute_force to fuse with the z r2c; any other use of the density mesh
+  // runs it first (pm_materialize)
+  bool density_pending = false;
 };
 
 namespace fs2 {
@@ -710,6 +715,25 @@ int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src) {
   });
 }
 
+namespace {
+float pm_fac(const fs2_pm* pm) {   // pm.cpp:78
+  return (float)(pm->pm_factor * pm->pm_factor * pm->pm_factor);
+}
+
+// Writes the density cells a compute_density left pending (same kernel and
+// records, so the same values as painting at once).
+void pm_materialize(fs2_pm* pm) {
+  if (!pm->density_pending) return;
+  pm->density_pending = false;
+  paint_tiled_cells((int64_t)pm->binned_np, pm->mesh, pm->nc, pm->boxsize_f, pm_fac(pm), true,
+                    pm->paint, pm->tiles);
+}
+
+void grid_sync(fs2_grid* g) {
+  if (g->owner) pm_materialize(g->owner);
+}
+}  // namespace
+
 // ---- grids -------------------------------------------------------------------------------------------------
 int fs2_grid_alloc(int nc, fs2_grid** out) {
   return guard([&] { *out = grid_new(nc); });
@@ -725,6 +749,7 @@ int fs2_grid_nc(const fs2_grid* g, int* nc) {
 }
 int fs2_grid_set_test_data(fs2_grid* g) {
   return guard([&] {
+    grid_sync(g);
     // py/py_fft.cpp:57-85: fx[ix,iy,iz] = (ix nc + iy) nc + iz + 1
     const int n = g->nc;
     std::vector<float> h((size_t)n * n * n);
@@ -733,13 +758,14 @@ int fs2_grid_set_test_data(fs2_grid* g) {
   });
 }
 int fs2_grid_get_real(fs2_grid* g, float* out) {
-  return guard([&] { grid_get_real(g, out); });
+  return guard([&] { grid_sync(g); grid_get_real(g, out); });
 }
 int fs2_grid_set_real(fs2_grid* g, const float* in) {
-  return guard([&] { grid_set_real(g, in); });
+  return guard([&] { grid_sync(g); grid_set_real(g, in); });
 }
 int fs2_grid_get_complex(fs2_grid* g, float* out) {
   return guard([&] {
+    grid_sync(g);
     const int n = g->nc;
     const size_t w = (size_t)(n / 2 + 1) * 8;
     FS2_HIP(hipMemcpy2DAsync(out, w, g->d, (size_t)grid_cpitch(n) * 8, w, (size_t)n * n,
@@ -749,6 +775,7 @@ int fs2_grid_get_complex(fs2_grid* g, float* out) {
 }
 int fs2_grid_set_complex(fs2_grid* g, const float* in) {
   return guard([&] {
+    grid_sync(g);
     const int n = g->nc;
     const size_t w = (size_t)(n / 2 + 1) * 8;
     FS2_HIP(hipMemcpy2DAsync(g->d, (size_t)grid_cpitch(n) * 8, in, w, w, (size_t)n * n,
@@ -757,10 +784,10 @@ int fs2_grid_set_complex(fs2_grid* g, const float* in) {
   });
 }
 int fs2_grid_r2c(fs2_grid* g) {
-  return guard([&] { fft_r2c_3d(g->d, g->nc); });
+  return guard([&] { grid_sync(g); fft_r2c_3d(g->d, g->nc); });
 }
 int fs2_grid_c2r(fs2_grid* g) {
-  return guard([&] { fft_c2r_3d(g->d, g->nc); });
+  return guard([&] { grid_sync(g); fft_c2r_3d(g->d, g->nc); });
 }
 
 // ---- PM ---------------------------------------------------------------------------------------------------------
@@ -790,6 +817,7 @@ int fs2_pm_create(int nc_pm, double pm_factor, double boxsize, fs2_pm** out) {
     pm->density.nc = nc_pm;
     pm->density.d = pm->mesh;
     pm->density.owning = false;
+    pm->density.owner = pm;
     *out = pm;
   });
 }
@@ -831,15 +859,21 @@ int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density) {
   return guard([&] {
     check_p(p);
     const int n = pm->nc;
-    const float fac = (float)(pm->pm_factor * pm->pm_factor * pm->pm_factor);   // pm.cpp:78
+    const float fac = pm_fac(pm);
+    pm->density_pending = false;
     if (pm->variant == FS2_PAINT_GLOBAL) {
       // pm.cpp:414-426 clear to -1, then the scatter of pm.cpp:57-139
       fill_grid(pm->mesh, n, -1.0f);
       paint_atomic(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending);
     } else {
+      const bool det = pm->variant == FS2_PAINT_SORTED;
       ensure_paint_scratch(pm->paint, p->np_local, paint_tiled_bins(n, pm->tiles));
-      paint_tiled(pos_of(p), pm->mesh, n, pm->boxsize_f, fac, p->wrap_pending,
-                  pm->variant == FS2_PAINT_SORTED, pm->paint, pm->tiles);
+      paint_tiled_bin(pos_of(p), n, pm->boxsize_f, p->wrap_pending, pm->paint, pm->tiles);
+      // the cells wait for compute_force to fuse them with the z r2c
+      pm->density_pending = det && paint_r2c_supported(n, pm->tiles);
+      if (!pm->density_pending)
+        paint_tiled_cells((int64_t)p->np_local, pm->mesh, n, pm->boxsize_f, fac, det, pm->paint,
+                          pm->tiles);
       if (p->wrap_pending) touch_x(p);   // the binning kernel wrapped x in place
     }
     pm->binned_version = pm->variant == FS2_PAINT_GLOBAL ? 0 : p->x_version;
@@ -855,6 +889,7 @@ int fs2_pm_check_total_density(fs2_pm* pm, double* sum) {
   return guard([&] {
     if (pm->mesh_is_k || pm->status != PM_DENSITY_DONE)
       fail(FS2_ERR_RUNTIME, "density mesh not available for the total-density check");
+    pm_materialize(pm);
     const double s = grid_sum(pm->mesh, pm->nc);
     if (sum) *sum = s;
     const double n = pm->nc;
@@ -881,12 +916,20 @@ int fs2_pm_compute_force(fs2_pm* pm, fs2_particles* p) {
     pm->mesh_is_k = false;
     pm->status = PM_FORCE_DONE;   // the density mesh no longer holds delta
     if (reuse_bins && column_readout_supported(n)) {
-      // the z c2r of the three force meshes fused into the readout
-      fft_force_spectra(pm->mesh, pm->force, n, f1);
+      // the density's tile kernel fused with its z r2c, and the z c2r of the
+      // three force meshes fused into the readout
+      const bool z_done = pm->density_pending;
+      if (z_done) {
+        paint_tiled_r2c((int64_t)pm->binned_np, pm->mesh, n, pm->boxsize_f, pm_fac(pm),
+                        pm->paint, pm->tiles);
+        pm->density_pending = false;
+      }
+      fft_force_spectra(pm->mesh, pm->force, n, f1, z_done);
       readout_columns(pm->force[0], pm->force[1], pm->force[2], p->f(), (int64_t)p->np_alloc,
                       (int64_t)p->np_local, n, pm->boxsize_f, pm->paint, pm->tiles);
       pm->force_spectral = true;
     } else {
+      pm_materialize(pm);
       fft_force_meshes(pm->mesh, pm->force, n, f1);
       pm->force_spectral = false;
       if (reuse_bins) {
@@ -928,6 +971,7 @@ int fs2_pm_power_spectrum(fs2_pm* pm, int nbins, double dk_bin, double* k, doubl
     if (pm->status != PM_DENSITY_DONE && !pm->mesh_is_k)
       fail(FS2_ERR_RUNTIME, "no density mesh for the power spectrum");
     if (!pm->mesh_is_k) {
+      pm_materialize(pm);
       fft_r2c_3d(pm->mesh, pm->nc);
       pm->mesh_is_k = true;
     }
@@ -944,6 +988,7 @@ int fs2_pm_power_spectrum_noalloc(fs2_pm* pm, int nbins, double dk_bin, double* 
     if (pm->status != PM_DENSITY_DONE && !pm->mesh_is_k)
       fail(FS2_ERR_RUNTIME, "no density mesh for the power spectrum");
     if (!pm->mesh_is_k) {
+      pm_materialize(pm);   // kernels only: no allocation
       fft_r2c_3d(pm->mesh, pm->nc);   // twiddles built by fs2_pm_create
       pm->mesh_is_k = true;
     }

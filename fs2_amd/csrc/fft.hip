@@ -351,11 +351,7 @@ k_fft_z_r2c(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
-    const float2 A = sp[L::at(k)];
-    const float2 Bc = cconj(sp[L::at(k == 0 ? 0 : M - k)]);
-    const float2 E = cscale(cadd(A, Bc), 0.5f);
-    const float2 O = cscale(csub(A, Bc), 0.5f);
-    const float2 X = cadd(E, mul_negi(cmul(twN[k], O)));
+    const float2 X = r2c_split(sp[L::at(k)], sp[L::at(k == 0 ? 0 : M - k)], twN[k]);
     if (valid) r[k] = X;
   }
   if (t == 0 && valid) {
@@ -403,11 +399,7 @@ k_fft_z_r2c_lt(float2* data, int64_t nrows, int cpitch, const float2* __restrict
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
-    const float2 A = sp[L::at(k)];
-    const float2 Bc = cconj(sp[L::at(k == 0 ? 0 : M - k)]);
-    const float2 E = cscale(cadd(A, Bc), 0.5f);
-    const float2 O = cscale(csub(A, Bc), 0.5f);
-    const float2 X = cadd(E, mul_negi(cmul(stN[k], O)));
+    const float2 X = r2c_split(sp[L::at(k)], sp[L::at(k == 0 ? 0 : M - k)], stN[k]);
     if (valid) r[k] = X;
   }
   if (t == 0 && valid) {
@@ -625,11 +617,12 @@ struct Impl {
 
   // force_meshes up to the z c2r: out[a] left holding the half-complex
   // spectra after the x and y inverse passes
-  static void force_spectra(float* mesh, float* const out[3], float f1, int nb) {
+  static void force_spectra(float* mesh, float* const out[3], float f1, int nb,
+                            bool z_done = false) {
     const int64_t nh = grid_cpitch(n);
     float2* c = (float2*)mesh;
     GreenArg g{0, n, 0.f};
-    z_r2c(mesh, nb);
+    if (!z_done) z_r2c(mesh, nb);
     pencil<false, false>("fft_y_fwd", c, c, nh, (int64_t)n * nh, g, nb);
     {
       const int ncols = n / 2 + 1;
@@ -1032,9 +1025,9 @@ void fft_force_meshes(float* mesh, float* const out[3], int n, float f1, int nb)
   dispatch(n, [&](auto impl) { decltype(impl)::force_meshes(mesh, out, f1, nb); });
 }
 
-void fft_force_spectra(float* mesh, float* const out[3], int n, float f1) {
+void fft_force_spectra(float* mesh, float* const out[3], int n, float f1, bool z_done) {
   if (!tuned(n)) fail(FS2_ERR_VALUE, "force spectra: %d is not a tuned transform size", n);
-  dispatch(n, [&](auto impl) { decltype(impl)::force_spectra(mesh, out, f1, 1); });
+  dispatch(n, [&](auto impl) { decltype(impl)::force_spectra(mesh, out, f1, 1, z_done); });
 }
 
 void fft_force_finish(float* const out[3], int n) {

@@ -38,6 +38,18 @@ __device__ __forceinline__ float2 c2r_merge(float2 A, float2 B, float2 tw, bool 
   return cconj(cadd(E, mul_posi(O)));
 }
 
+// r2c along z: a row of n = 2M reals read as M complex z[m] = x[2m] + i
+// x[2m+1] and transformed (Z = DFT_M z) is split into the half spectrum:
+// X[k] = E - i W^k O,  E = (Z[k] + conj Z[M-k])/2,  O = (Z[k] - conj Z[M-k])/2,
+// from A = Z[k], B = Z[M - k] (Z[0] for k = 0) and tw = W^k; X[M] = Re Z[0]
+// - Im Z[0].
+__device__ __forceinline__ float2 r2c_split(float2 A, float2 B, float2 tw) {
+  const float2 Bc = cconj(B);
+  const float2 E = cscale(cadd(A, Bc), 0.5f);
+  const float2 O = cscale(csub(A, Bc), 0.5f);
+  return cadd(E, mul_negi(cmul(tw, O)));
+}
+
 // ---- radix-R DFTs, forward sign ----------------------------------------------------
 template <int R> struct DFT;
 

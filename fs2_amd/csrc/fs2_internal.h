@@ -126,7 +126,8 @@ void fft_force_meshes(float* mesh, float* const out[3], int n, float f1, int nb 
 // fft_force_meshes in two parts (tuned sizes): the spectra of the three force
 // meshes after the x and y inverse passes (half-complex [x][y][kz] rows),
 // and the three z c2r passes that turn them into the real meshes.
-void fft_force_spectra(float* mesh, float* const out[3], int n, float f1);
+// z_done: the mesh already holds the z r2c of the density (paint_tiled_r2c).
+void fft_force_spectra(float* mesh, float* const out[3], int n, float f1, bool z_done = false);
 void fft_force_finish(float* const out[3], int n);
 // The FP32 twiddle table e^{-2 pi i k/N}, k < N, of the transforms.
 const float2* fft_twiddle_table(int N);
@@ -173,6 +174,20 @@ void ensure_paint_scratch(PaintScratch& s, size_t np, size_t nbins);   // grows 
 void free_paint_scratch(PaintScratch& s);
 void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
                  bool deterministic, const PaintScratch& s, const TileOpts& o = TileOpts());
+// paint_tiled in two parts: the binning (bin counts, scan, scatter of the
+// positions into bin-ordered records) and the tile kernel that writes the
+// cells from those records.
+void paint_tiled_bin(PosArgs p, int n, float boxsize, bool wrap, const PaintScratch& s,
+                     const TileOpts& o);
+void paint_tiled_cells(int64_t np, float* grid, int n, float boxsize, float fac,
+                       bool deterministic, const PaintScratch& s, const TileOpts& o);
+// The deterministic tile kernel fused with the z r2c pass of the force
+// transform (tiles of full z rows, one row per wavefront): grid receives the
+// z-transformed density, exactly what paint_tiled_cells followed by the z
+// r2c pass writes, without the real density.
+bool paint_r2c_supported(int n, const TileOpts& o);
+void paint_tiled_r2c(int64_t np, float* grid, int n, float boxsize, float fac,
+                     const PaintScratch& s, const TileOpts& o);
 // Readout of the three force meshes at the particles binned by the last
 // paint_tiled call (positions unchanged since); forces land at f[i + a*fstride].
 void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, int64_t fstride,

@@ -498,27 +498,20 @@ __device__ __forceinline__ void store_group(float* p, const float* v) {
   }
 }
 
-// K4: one workgroup per tile.  ACC is float (LDS float atomics) or
-// unsigned long long (int64 fixed point, deterministic).
+// Accumulates every CIC corner that lands in tile tc into acc (LDS, zeroed by
+// the caller), from the tile's own particles and from the spilling classes
+// of its lower neighbours (tile_source_ranges; rb, rpre, nranges are its
+// shared arrays).  ACC is float (LDS float atomics) or unsigned long long
+// (int64 fixed point, deterministic).  Ends with a barrier.
 template <typename ACC>
-__global__ void __launch_bounds__(kTileBlock)
-k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
-             TileGeom g, float fac, float fix, double fix_inv, float* __restrict__ grid) {
-  extern __shared__ unsigned char smem[];
-  ACC* acc = (ACC*)smem;
-  const int tcells = g.tx * g.ty * g.tz;
-  lds_zero(smem, tcells * (int)sizeof(ACC));
-
-  // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
-  const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
-  offsets += (int64_t)tc.box * g.bins_box;
+__device__ __forceinline__ void tile_deposit(ACC* acc, const float4* __restrict__ rec,
+                                             const unsigned* __restrict__ offsets,
+                                             const TileGeom& g, const TileCoord& tc, float fac,
+                                             float fix, unsigned* rb, unsigned* rpre,
+                                             int* nranges) {
   const int x0 = tc.Tx * g.tx, y0 = tc.Ty * g.ty, z0 = tc.Tz * g.tz;
-  float* gt = grid + (int64_t)tc.box * g.mstride + ((int64_t)x0 * g.n + y0) * g.ncz + z0;
-
-  __shared__ unsigned rb[64], rpre[65];
-  __shared__ int nranges;
-  tile_source_ranges(offsets, g, tc.Tx, tc.Ty, tc.Tz, rb, rpre, &nranges);
-  const int nr = nranges;
+  tile_source_ranges(offsets, g, tc.Tx, tc.Ty, tc.Tz, rb, rpre, nranges);
+  const int nr = *nranges;
   const unsigned total = rpre[nr];
   // fixed point: fix is a power of two, so (fac fix) wx wy wz rounds exactly
   // as (fac wx wy wz) fix
@@ -576,19 +569,105 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
       if (ok[u]) deposit(q[u]);
   }
   __syncthreads();
+}
+
+// The value a tile cell stores: its CIC sum minus 1 (pm.cpp:414-426 clears to
+// -1 before the adds).
+template <typename ACC>
+__device__ __forceinline__ float cell_value(ACC a, double fix_inv) {
+  if constexpr (sizeof(ACC) == 4) {
+    return a - 1.0f;
+  } else {
+    return (float)((double)(long long)a * fix_inv) - 1.0f;
+  }
+}
+
+// K4: one workgroup per tile.
+template <typename ACC>
+__global__ void __launch_bounds__(kTileBlock)
+k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
+             TileGeom g, float fac, float fix, double fix_inv, float* __restrict__ grid) {
+  extern __shared__ unsigned char smem[];
+  ACC* acc = (ACC*)smem;
+  const int tcells = g.tx * g.ty * g.tz;
+  lds_zero(smem, tcells * (int)sizeof(ACC));
+
+  // z-adjacent tiles (consecutive indices) on one XCD share halo lines in L2
+  const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
+  offsets += (int64_t)tc.box * g.bins_box;
+  const int x0 = tc.Tx * g.tx, y0 = tc.Ty * g.ty, z0 = tc.Tz * g.tz;
+  float* gt = grid + (int64_t)tc.box * g.mstride + ((int64_t)x0 * g.n + y0) * g.ncz + z0;
+
+  __shared__ unsigned rb[64], rpre[65];
+  __shared__ int nranges;
+  tile_deposit<ACC>(acc, rec, offsets, g, tc, fac, fix, rb, rpre, &nranges);
   for_tile_groups(g, [&](int c, int off, auto vc) {
     constexpr int V = decltype(vc)::value;
     float o[V];
 #pragma unroll
-    for (int k = 0; k < V; ++k) {
-      if constexpr (sizeof(ACC) == 4) {
-        o[k] = acc[c + k] - 1.0f;
-      } else {
-        o[k] = (float)((double)(long long)acc[c + k] * fix_inv) - 1.0f;
-      }
-    }
+    for (int k = 0; k < V; ++k) o[k] = cell_value<ACC>(acc[c + k], fix_inv);
     store_group<V>(gt + off, o);
   });
+}
+
+// K4 fused with the z r2c of the force transform (1024^3 meshes, tiles of
+// kTileBlock / 64 full z rows): after the deterministic deposit, wavefront w
+// takes row w of the tile, forms its cell values exactly as k_tile_paint
+// stores them, transforms the row (its own int64 accumulators serve as the
+// transform's LDS scratch) and writes the half spectrum, as the z r2c pass
+// would from the stored mesh.  The real density is never written.
+template <int M>
+__global__ void __launch_bounds__(kTileBlock)
+k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
+                 TileGeom g, float fac, float fix, double fix_inv,
+                 const float2* __restrict__ twM, const float2* __restrict__ twN,
+                 float2* __restrict__ grid) {
+  using namespace fftdev;
+  using ACC = unsigned long long;
+  constexpr int T = Plan<M>::T;
+  static_assert(T == 64, "one wavefront per z row");
+  using L = ContigLayout;
+  static_assert(L::size(M) * sizeof(float2) <= 2 * M * sizeof(ACC), "row scratch fits a row");
+  extern __shared__ unsigned char smem[];
+  ACC* acc = (ACC*)smem;
+  __shared__ float2 stM[M];
+  __shared__ float2 stN[M];
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    stM[i] = twM[i];
+    stN[i] = twN[i];
+  }
+  const int tcells = g.tx * g.ty * g.tz;   // tz = n = 2M: tx * ty rows
+  lds_zero(smem, tcells * (int)sizeof(ACC));
+  const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
+  __shared__ unsigned rb[64], rpre[65];
+  __shared__ int nranges;
+  tile_deposit<ACC>(acc, rec, offsets, g, tc, fac, fix, rb, rpre, &nranges);
+  // row w of the tile: cells (x0 + w / ty, y0 + w % ty, all z)
+  const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+  const int lx = w >> g.ly, ly = w & (g.ty - 1);
+  const ACC* ar = acc + (w << g.lz);
+  float2 z[M / T];
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int m = t + i * T;
+    z[i] = make_float2(cell_value<ACC>(ar[2 * m], fix_inv), cell_value<ACC>(ar[2 * m + 1], fix_inv));
+  }
+  float2* sp = (float2*)(acc + (w << g.lz));
+  fft_regs<M, L, WaveSync>(z, sp, t, stM);   // starts with a wavefront barrier
+  WaveSync::sync();   // the row's lanes are done reading the last stage's input
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = z[i];
+  WaveSync::sync();
+  float2* r = grid + ((int64_t)(tc.Tx * g.tx + lx) * g.n + (tc.Ty * g.ty + ly)) * (g.ncz / 2);
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int k = t + i * T;
+    r[k] = r2c_split(sp[L::at(k)], sp[L::at(k == 0 ? 0 : M - k)], stN[k]);
+  }
+  if (t == 0) {
+    const float2 A = sp[L::at(0)];
+    r[M] = make_float2(A.x - A.y, 0.f);
+  }
 }
 
 // Copies the (tx+1) x (ty+1) x (tz+1) halo block of NM meshes starting at
@@ -1089,14 +1168,11 @@ size_t paint_tiled_bins(int n, const TileOpts& o) {
   return (size_t)g.nbox * g.bins_box;
 }
 
-void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
-                 bool deterministic, const PaintScratch& s, const TileOpts& o) {
+void paint_tiled_bin(PosArgs p, int n, float boxsize, bool wrap, const PaintScratch& s,
+                     const TileOpts& o) {
   const TileGeom g = make_geom(n, boxsize, o, p.np);
   check_scratch(g, p.np, s);
   const int64_t nbins = (int64_t)g.nbox * g.bins_box;
-  const int64_t nblk = (int64_t)g.nbox * g.ntiles;
-  const int tcells = g.tx * g.ty * g.tz;
-  check_lds(tcells * (deterministic ? sizeof(unsigned long long) : sizeof(float)), "paint");
   launch("paint_clear_bins", (double)nbins * 4.0, k_fill_u32, dim3(blocks_for(nbins)),
          dim3(kBlock), 0, s.counts, nbins, 0u);
   if (p.np > 0) {
@@ -1117,22 +1193,66 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
     launch("paint_scatter", (double)p.np * 28.0, k_scatter, dim3(bin_blocks(p.np)),
            dim3(kBinBlock), 0, (const float*)p.x, p.stride, p.np, g, s.cursor, s.rec);
   }
-  const double bytes = (double)p.np * 12.0 + (double)n * n * n * 4.0 * g.nbox;
-  // fixed-point unit 2^-k with fac * 2^k <= 2^30, so every term fits an int32
+}
+
+namespace {
+// fixed-point unit 2^-k with fac * 2^k <= 2^30, so every term fits an int32
+struct FixedPoint {
+  float fix;
+  double fix_inv;
+};
+FixedPoint fixed_point(float fac) {
   int e = 0;
   std::frexp((double)std::max(fac, 1e-30f), &e);   // fac < 2^e
   const int k = std::min(60, 30 - e);
-  const float fix = (float)std::ldexp(1.0, k);
-  const double fix_inv = std::ldexp(1.0, -k);
+  return {(float)std::ldexp(1.0, k), std::ldexp(1.0, -k)};
+}
+}  // namespace
+
+void paint_tiled_cells(int64_t np, float* grid, int n, float boxsize, float fac,
+                       bool deterministic, const PaintScratch& s, const TileOpts& o) {
+  const TileGeom g = make_geom(n, boxsize, o, np);
+  check_scratch(g, np, s);
+  const int64_t nblk = (int64_t)g.nbox * g.ntiles;
+  const int tcells = g.tx * g.ty * g.tz;
+  check_lds(tcells * (deterministic ? sizeof(unsigned long long) : sizeof(float)), "paint");
+  const double bytes = (double)np * 12.0 + (double)n * n * n * 4.0 * g.nbox;
+  const FixedPoint fp = fixed_point(fac);
   if (deterministic) {
     launch("paint_tile_det", bytes, k_tile_paint<unsigned long long>, dim3((unsigned)nblk),
            dim3(kTileBlock), tcells * sizeof(unsigned long long), (const float4*)s.rec,
-           (const unsigned*)s.offsets, g, fac, fix, fix_inv, grid);
+           (const unsigned*)s.offsets, g, fac, fp.fix, fp.fix_inv, grid);
   } else {
     launch("paint_tile", bytes, k_tile_paint<float>, dim3((unsigned)nblk), dim3(kTileBlock),
            tcells * sizeof(float), (const float4*)s.rec, (const unsigned*)s.offsets,
-           g, fac, fix, fix_inv, grid);
+           g, fac, fp.fix, fp.fix_inv, grid);
   }
+}
+
+void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool wrap,
+                 bool deterministic, const PaintScratch& s, const TileOpts& o) {
+  paint_tiled_bin(p, n, boxsize, wrap, s, o);
+  paint_tiled_cells(p.np, grid, n, boxsize, fac, deterministic, s, o);
+}
+
+bool paint_r2c_supported(int n, const TileOpts& o) {
+  return n == 1024 && o.tx * o.ty * 64 == kTileBlock && o.tz == n && o.nbox == 1;
+}
+
+void paint_tiled_r2c(int64_t np, float* grid, int n, float boxsize, float fac,
+                     const PaintScratch& s, const TileOpts& o) {
+  if (!paint_r2c_supported(n, o)) fail(FS2_ERR_VALUE, "fused paint + r2c: mesh %d", n);
+  const TileGeom g = make_geom(n, boxsize, o, np);
+  check_scratch(g, np, s);
+  constexpr int M = 512;
+  const int tcells = g.tx * g.ty * g.tz;
+  check_lds(tcells * sizeof(unsigned long long) + 2 * M * sizeof(float2), "paint + r2c");
+  // algorithmic bytes: positions read, the half spectrum written
+  const double bytes = (double)np * 12.0 + (double)n * n * (M + 1) * 8.0;
+  const FixedPoint fp = fixed_point(fac);
+  launch("paint_tile_r2c", bytes, k_tile_paint_r2c<M>, dim3((unsigned)g.ntiles), dim3(kTileBlock),
+         tcells * sizeof(unsigned long long), (const float4*)s.rec, (const unsigned*)s.offsets, g,
+         fac, fp.fix, fp.fix_inv, fft_twiddle_table(M), fft_twiddle_table(2 * M), (float2*)grid);
 }
 
 void readout_tiled(const float* g0, const float* g1, const float* g2, float* f, int64_t fstride,
@@ -1151,10 +1271,10 @@ bool column_readout_supported(int n) { return n == 1024; }
 
 TileOpts column_tile_opts(int n) {
   TileOpts o;
-  if (column_readout_supported(n)) {
+  if (column_readout_supported(n)) {   // full z rows: the paint can fuse the z r2c too
     o.tx = kColX;
     o.ty = kColY;
-    o.tz = 512;
+    o.tz = n;
   }
   return o;
 }

@@ -340,12 +340,15 @@ def test_tiled_readout_matches_gather_readout(fs, n_p, nc):
 
 @pytest.mark.timeout(600)
 def test_column_readout_matches_unfused_path(fs):
-    """At 1024^3 compute_force fuses the z c2r of the three force meshes into
-    the readout (paint bins in 2 x 4 x 512 column tiles, rows transformed in
-    LDS, no real force mesh written).  Forces must equal, bit for bit, those
-    of the unfused path (z c2r passes writing the real meshes, per-particle
-    gather readout), and force_mesh must return the same real meshes either
-    way (finished on demand after the fused path)."""
+    """At 1024^3 pm.force fuses the density's tile kernel with the z r2c
+    (compute_density leaves the cells pending; tiles of 2 x 4 full z rows)
+    and the z c2r of the three force meshes with the readout (rows
+    transformed in LDS, no real force mesh written).  Forces must equal, bit
+    for bit, those of the path with the density written first (read back
+    between compute_density and compute_force) and those of the unfused path
+    (z c2r passes writing the real meshes, per-particle gather readout), and
+    force_mesh must return the same real meshes each way (finished on demand
+    after the fused path)."""
     _announce("column readout vs unfused path, 128^3 particles, 1024^3 mesh")
     n_p, nc, L = 128, 1024, 1024.0
     x = np.mod(_lattice_positions(n_p, L, 2.0, seed=17), L)
@@ -360,6 +363,15 @@ def test_column_readout_matches_unfused_path(fs):
     f_col = p.force
     assert np.isfinite(f_col).all() and np.abs(f_col).max() > 0
     mesh_col = [fs.pm.force_mesh(a)[::7, 3::11, :].copy() for a in range(3)]
+    # density written by the plain tile kernel (read back), then the force
+    p.set_times(0.5, 0.5, -1.0)
+    fs.pm.send_positions(p)
+    d = fs.pm.compute_density(p)
+    rho = d.asarray()[::5, 2::9, :].copy()
+    fs.pm.compute_force(p)
+    assert np.array_equal(p.force, f_col), np.abs(p.force - f_col).max()
+    # the density of the clump region: painted, not pending
+    assert rho.max() > 0.0 and np.isclose(rho.min(), -1.0)
     p.set_times(0.5, 0.5, -1.0)
     fs.pm.send_positions(p)
     fs.pm.compute_density(p)
