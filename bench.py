@@ -324,14 +324,32 @@ def copy_probe(fs, gib=4):
 # Compulsory bytes of one COLA step by stage (Np particles, Ng = n^3,
 # Nk = n^2 (n/2+1)): the paint reads x and writes delta once; the FFT stage
 # is one r2c and three Green-multiplied c2r transforms, each reading and
-# writing its mesh once; the readout reads x and three meshes and writes f;
-# kick and drift each read and write 60 bytes per particle.
-STAGES = {
-    "paint": (("paint_",), lambda Np, Ng, Nk: 12 * Np + 4 * Ng),
-    "fft": (("fft_",), lambda Np, Ng, Nk: 16 * Ng + 32 * Nk),
-    "readout": (("readout",), lambda Np, Ng, Nk: 24 * Np + 12 * Ng),
-    "kick_drift": (("cola_",), lambda Np, Ng, Nk: 120 * Np),
-}
+# writing its mesh once (4 Ng + 8 Nk); the readout reads x and three meshes
+# and writes f; kick and drift each read and write 60 bytes per particle.
+# Where a kernel fuses work of two stages (the paint's tile kernel with the
+# z r2c, the three z c2r with the readout), the stage it is timed in also
+# carries the compulsory bytes of the transforms it contains, and the FFT
+# stage keeps what is left (nothing once both ends are fused: its y and x
+# passes then move only intermediate meshes).  The stage bytes always sum
+# to 156 Np + 32 Ng + 32 Nk.
+def stage_table(prof):
+    names = {r[0] for r in prof}
+    r2c_in_paint = "paint_tile_r2c" in names
+    c2r_in_readout = "readout_columns" in names
+    xform = lambda Ng, Nk: 4 * Ng + 8 * Nk
+    return {
+        "paint+z_r2c" if r2c_in_paint else "paint": (
+            ("paint_",),
+            lambda Np, Ng, Nk: 12 * Np + 4 * Ng + (xform(Ng, Nk) if r2c_in_paint else 0)),
+        "fft": (
+            ("fft_",),
+            lambda Np, Ng, Nk: 16 * Ng + 32 * Nk - (xform(Ng, Nk) if r2c_in_paint else 0)
+            - (3 * xform(Ng, Nk) if c2r_in_readout else 0)),
+        "z_c2r+readout" if c2r_in_readout else "readout": (
+            ("readout",),
+            lambda Np, Ng, Nk: 24 * Np + 12 * Ng + (3 * xform(Ng, Nk) if c2r_in_readout else 0)),
+        "kick_drift": (("cola_",), lambda Np, Ng, Nk: 120 * Np),
+    }
 
 
 def stage_roofline(prof, steps, npart, n):
@@ -339,11 +357,12 @@ def stage_roofline(prof, steps, npart, n):
     the fraction of 8 TB/s its compulsory bytes reach."""
     Ng, Nk = n**3, n * n * (n // 2 + 1)
     out = {}
-    for stage, (prefixes, nbytes) in STAGES.items():
+    for stage, (prefixes, nbytes) in stage_table(prof).items():
         ms = sum(r[2] for r in prof if r[0].startswith(prefixes)) / steps
         b = nbytes(npart, Ng, Nk)
         out[stage] = {"ms": round(ms, 4), "algorithmic_bytes": b,
-                      "frac": round(b / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if ms > 0 else None}
+                      "frac": round(b / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+                      if ms > 0 and b > 0 else None}
     return out
 
 
