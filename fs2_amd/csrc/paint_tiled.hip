@@ -787,17 +787,120 @@ k_tile_readout(const float4* __restrict__ rec, const unsigned* __restrict__ offs
 // kColX + 1 rows it shares with the next column: the row's LDS slice serves
 // as the transform's scratch and then holds the real row.  It then reads
 // the force out at the column's particles.  The real force meshes are never
-// written; each spectrum row is read and transformed ~1.5 times on
-// average.  Particles come from the bins of a paint with kColX x kColY x tz
-// tiles (z fastest, so a column's bins are one contiguous range).  The row
-// transforms are the z c2r pass's arithmetic and the interpolation is
-// k_tile_readout's, so forces are bitwise those of the unfused path.
-constexpr int kColX = 2, kColY = 4;
-constexpr int kColRows = (kColX + 1) * (kColY + 1);
+// written; a strip of 8 columns reads 15 + 7 x 10 spectrum rows for its 64
+// owned ones (1.33 times each).  Particles come from the bins of a paint
+// with kColX x kColY x tz tiles (z fastest, so a column's bins are one
+// contiguous range).  The row transforms are the z c2r pass's arithmetic
+// and the interpolation is k_tile_readout's, so forces are bitwise those of
+// the unfused path.
+//
+// Software pipeline over the 3 x kColStrip (axis, column) units: each
+// wavefront's first row of unit u + 1 is loaded into registers while unit
+// u's particles are read out, and unit u's first record per thread is
+// loaded before its rows are transformed, so neither load's latency stands
+// alone between barriers.  Rows are dealt to the kColWaves wavefronts
+// round-robin: a later column's 10 new rows take one each, the strip's
+// first column (15 rows) gives three wavefronts a second row, loaded in
+// place.  Measured at 512^3 particles / 1024^3 on the clustered run
+// (readout ms per launch, 10-step average): 4 x 2 columns with 12
+// wavefronts 6.2 (6.5 without the prefetch); 15 wavefronts 6.7-6.9, 10
+// wavefronts 7.7; 2 x 4 columns 6.4-6.5; strips of 4 or 16 columns slower
+// than 8.
+constexpr int kColX = 4, kColY = 2;
+constexpr int kColHY = kColY + 1;                 // y rows of a column: a ring of slices
+constexpr int kColRows = (kColX + 1) * kColHY;    // row slices
+constexpr int kColNew = (kColX + 1) * kColY;      // rows each later column of a strip adds
 constexpr int kColStrip = 8;
+constexpr int kColWaves = 12;
+constexpr int kColUnits = 3 * kColStrip;
+static_assert(kColWaves >= kColNew && 2 * kColWaves >= kColRows, "rows per wavefront");
 
+__device__ __forceinline__ void column_row(const float2* s0, const float2* s1, const float2* s2,
+                                           int u, int r, int x0, int Sy, int n, int cp,
+                                           const float2*& src, int& slot) {
+  const int a = u / kColStrip, c = u - a * kColStrip;
+  int hx, hy;
+  if (c == 0) {
+    hx = r / kColHY;
+    hy = r - hx * kColHY;
+  } else {
+    hx = r / kColY;
+    hy = 1 + (r - hx * kColY);
+  }
+  int gx = x0 + hx, gy = (Sy * kColStrip + c) * kColY + hy;
+  if (gx >= n) gx -= n;
+  if (gy >= n) gy -= n;
+  const float2* sa = s0;
+  if (a == 1) sa = s1;
+  if (a == 2) sa = s2;
+  src = sa + ((int64_t)gx * n + gy) * cp;
+  slot = hx * kColHY + (kColY * c + hy) % kColHY;
+}
+
+// The row's X[k] and X[M - k] for k = t + i T (X[M] for k = 0): one base per
+// half and immediate offsets per element (the lane index is made opaque so
+// that the compiler does not hoist 16 addresses out of the caller's loop).
+template <int M, int T>
+__device__ __forceinline__ void column_load_row(const float2* __restrict__ src, int t0,
+                                                float2* pa, float2* pb) {
+  int t = t0;
+  asm volatile("" : "+v"(t));
+  const float2* qa = src + t;
+  const float2* qb = src + (M - t);
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    pa[i] = qa[i * T];
+    pb[i] = qb[-i * T];
+  }
+}
+
+// z c2r of one row from its loaded halves into LDS slice sp: the real row,
+// x[2m] + i x[2m+1] = conj(z[m]), and x[n] = x[0] after it, so a stencil's
+// z + 1 corner is always the next float.
+template <int M, int T, class L>
+__device__ __forceinline__ void column_transform_row(const float2* pa, const float2* pb,
+                                                     float2* sp, int t,
+                                                     const float2* stM, const float2* stN) {
+  using namespace fftdev;
+  float2 z[M / T];
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {
+    const int k = t + i * T;
+    z[i] = c2r_merge(pa[i], pb[i], stN[k], k == 0);
+  }
+  fft_regs<M, L, WaveSync>(z, sp, t, stM);
+  WaveSync::sync();   // the row's lanes are done reading the last stage's input
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) sp[t + i * T] = cconj(z[i]);
+  if (t == 0) ((float*)sp)[2 * M] = z[0].x;
+}
+
+// Force component at one record from the column's real rows h (slices RS
+// floats apart, ring of kColHY y rows per x row): k_tile_readout's
+// interpolation, written to the particle's id-order slot.
+template <int RS>
+__device__ __forceinline__ void column_readout_one(const float* h, const float4 r4, float dx_inv,
+                                                   int n, int x0, int y0, int ybase, float* fa) {
+  const Cic1 cx = cic1(r4.x, dx_inv, n);
+  const Cic1 cy = cic1(r4.y, dx_inv, n);
+  const Cic1 cz = cic1(r4.z, dx_inv, n);
+  const int lx = cx.i0 - x0, ly = ybase + (cy.i0 - y0);
+  const int q00 = (lx * kColHY + ly % kColHY) * RS;
+  const int q01 = (lx * kColHY + (ly + 1) % kColHY) * RS;
+  const int q10 = q00 + kColHY * RS, q11 = q01 + kColHY * RS;
+  const int z0 = cz.i0, z1 = cz.i0 + 1;
+  // pm.cpp:203-218: f = 0; f += (ix0 terms); f += (ix1 terms)
+  float v = 0.f;
+  v += h[q00 + z0] * cx.w0 * cy.w0 * cz.w0 + h[q00 + z1] * cx.w0 * cy.w0 * cz.w1 +
+       h[q01 + z0] * cx.w0 * cy.w1 * cz.w0 + h[q01 + z1] * cx.w0 * cy.w1 * cz.w1;
+  v += h[q10 + z0] * cx.w1 * cy.w0 * cz.w0 + h[q10 + z1] * cx.w1 * cy.w0 * cz.w1 +
+       h[q11 + z0] * cx.w1 * cy.w1 * cz.w0 + h[q11 + z1] * cx.w1 * cy.w1 * cz.w1;
+  fa[__float_as_uint(r4.w)] = v;
+}
+
+// two workgroups per CU (LDS), 6 wavefronts per SIMD
 template <int M>
-__global__ void __launch_bounds__(kColRows * 64) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(kColWaves * 64) __attribute__((amdgpu_waves_per_eu(6)))
 k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
                  const float2* __restrict__ s2, const float4* __restrict__ rec,
                  const unsigned* __restrict__ offsets, TileGeom g,
@@ -806,10 +909,11 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
   using namespace fftdev;
   constexpr int T = Plan<M>::T;
   static_assert(T == 64, "one wavefront per z row");
+  constexpr int I = M / T;         // elements per lane
   using L = ContigLayout;
   constexpr int LS = L::size(M);   // float2 per row slice: > M, a real row and one more float
   static_assert(LS > M, "row slice holds n + 1 floats");
-  constexpr int HY = kColY + 1;    // y rows of a column, a ring of HY slices per x row
+  constexpr int RS = 2 * LS;       // floats per row slice
   __shared__ float2 rows[kColRows * LS];
   __shared__ float2 stM[M];
   __shared__ float2 stN[M];
@@ -819,6 +923,8 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
   }
   const int n = g.n;
   const int cp = g.ncz / 2;
+  const float dx_inv = g.dx_inv;
+  const unsigned nty = (unsigned)g.nty, ntz = (unsigned)g.ntz;
   // strips x fastest: the strips next to each other in x share their
   // boundary rows and run together on one XCD (its L2 serves the second read)
   const int nsx = n / kColX;
@@ -827,76 +933,53 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
   const int x0 = Cx * kColX;
   const int w0 = threadIdx.x >> 6, t0 = threadIdx.x & 63;
   const float* h = (const float*)rows;
-  constexpr int RS = 2 * LS;   // floats per row slice
+
+  float2 pa[I], pb[I];
+  const float2* src;
+  int slot;
+  column_row(s0, s1, s2, 0, w0, x0, Sy, n, cp, src, slot);
+  column_load_row<M, T>(src, t0, pa, pb);
   __syncthreads();   // the twiddles are staged
 #pragma unroll 1
-  for (int a = 0; a < 3; ++a) {
-    const float2* sa = a == 0 ? s0 : a == 1 ? s1 : s2;
-#pragma unroll 1
-    for (int c = 0; c < kColStrip; ++c) {
-      const int Cy = Sy * kColStrip + c;
-      const int y0 = Cy * kColY;
-      // this wavefront's row: all HY x (kColX+1) rows for the strip's first
-      // column, then the kColY new ones per x row (the y0 row is the last
-      // column's y0 + kColY row, still in its slice)
-      int hx, hy;
-      bool busy;
-      if (c == 0) {
-        hx = w0 / HY;
-        hy = w0 - hx * HY;
-        busy = true;
-      } else {
-        hx = w0 / kColY;
-        hy = 1 + (w0 - hx * kColY);
-        busy = w0 < (kColX + 1) * kColY;
-      }
-      if (busy) {
-        // opaque copies of the lane and slice: without them the compiler
-        // keeps every LDS address of the transform live across the loops
-        int t = t0, slot = hx * HY + (kColY * c + hy) % HY;
-        asm volatile("" : "+v"(t), "+v"(slot));
-        float2* sp = rows + slot * LS;
-        int gx = x0 + hx, gy = y0 + hy;
-        if (gx >= n) gx -= n;
-        if (gy >= n) gy -= n;
-        const float2* r = sa + ((int64_t)gx * n + gy) * cp;
-        float2 z[M / T];
-#pragma unroll
-        for (int i = 0; i < M / T; ++i) {
-          const int k = t + i * T;
-          z[i] = c2r_merge(r[k], r[M - k], stN[k], k == 0);
-        }
-        fft_regs<M, L, WaveSync>(z, sp, t, stM);
-        WaveSync::sync();   // the row's lanes are done reading the last stage's input
-        // the real row, x[2m] + i x[2m+1] = conj(z[m]), and x[n] = x[0]
-        // after it, so a stencil's z + 1 corner is always the next float
-#pragma unroll
-        for (int i = 0; i < M / T; ++i) sp[t + i * T] = cconj(z[i]);
-        if (t == 0) ((float*)sp)[2 * M] = z[0].x;
-      }
-      __syncthreads();
-      const unsigned col = (unsigned)(Cx * g.nty + Cy) * (unsigned)g.ntz;
-      const unsigned pb = offsets[col * 8u], pe = offsets[(col + (unsigned)g.ntz) * 8u];
-      const int ybase = kColY * c;
-      for (unsigned j = pb + threadIdx.x; j < pe; j += blockDim.x) {
-        const float4 r4 = rec[j];
-        const Cic1 cx = cic1(r4.x, g.dx_inv, n);
-        const Cic1 cy = cic1(r4.y, g.dx_inv, n);
-        const Cic1 cz = cic1(r4.z, g.dx_inv, n);
-        const int lx = cx.i0 - x0, ly = ybase + (cy.i0 - y0);
-        const int q00 = (lx * HY + ly % HY) * RS, q01 = (lx * HY + (ly + 1) % HY) * RS;
-        const int q10 = q00 + HY * RS, q11 = q01 + HY * RS;
-        const int z0 = cz.i0, z1 = cz.i0 + 1;
-        // pm.cpp:203-218: f = 0; f += (ix0 terms); f += (ix1 terms)
-        float v = 0.f;
-        v += h[q00 + z0] * cx.w0 * cy.w0 * cz.w0 + h[q00 + z1] * cx.w0 * cy.w0 * cz.w1 +
-             h[q01 + z0] * cx.w0 * cy.w1 * cz.w0 + h[q01 + z1] * cx.w0 * cy.w1 * cz.w1;
-        v += h[q10 + z0] * cx.w1 * cy.w0 * cz.w0 + h[q10 + z1] * cx.w1 * cy.w0 * cz.w1 +
-             h[q11 + z0] * cx.w1 * cy.w1 * cz.w0 + h[q11 + z1] * cx.w1 * cy.w1 * cz.w1;
-        f[__float_as_uint(r4.w) + a * fstride] = v;
-      }
-      __syncthreads();   // the rows are read before the next column overwrites them
+  for (int u = 0; u < kColUnits; ++u) {
+    const int a = u / kColStrip, c = u - a * kColStrip;
+    const int Cy = Sy * kColStrip + c;
+    const int y0 = Cy * kColY;
+    const unsigned col = ((unsigned)Cx * nty + (unsigned)Cy) * ntz;
+    const unsigned pbeg = offsets[col * 8u], pend = offsets[(col + ntz) * 8u];
+    // this thread's first record of the column, in flight during the transform
+    const unsigned j0 = pbeg + threadIdx.x;
+    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j0 < pend) r0 = rec[j0];
+    const int nrows = c == 0 ? kColRows : kColNew;
+    int r = w0;
+    if (r < nrows) {   // the row loaded during the last unit
+      // opaque copies of the lane and slice: without them the compiler
+      // keeps every LDS address of the transform live across the loop
+      int t = t0, sl = slot;
+      asm volatile("" : "+v"(t), "+v"(sl));
+      column_transform_row<M, T, L>(pa, pb, rows + sl * LS, t, stM, stN);
+      r += kColWaves;
     }
+    for (; r < nrows; r += kColWaves) {   // the first column's second rows
+      column_row(s0, s1, s2, u, r, x0, Sy, n, cp, src, slot);
+      column_load_row<M, T>(src, t0, pa, pb);
+      int t = t0, sl = slot;
+      asm volatile("" : "+v"(t), "+v"(sl));
+      column_transform_row<M, T, L>(pa, pb, rows + sl * LS, t, stM, stN);
+    }
+    // the next unit's first row, in flight during this unit's readout
+    if (u + 1 < kColUnits && w0 < ((u + 1) % kColStrip == 0 ? kColRows : kColNew)) {
+      column_row(s0, s1, s2, u + 1, w0, x0, Sy, n, cp, src, slot);
+      column_load_row<M, T>(src, t0, pa, pb);
+    }
+    __syncthreads();
+    float* fa = f + a * fstride;
+    const int ybase = kColY * c;
+    if (j0 < pend) column_readout_one<RS>(h, r0, dx_inv, n, x0, y0, ybase, fa);
+    for (unsigned j = j0 + blockDim.x; j < pend; j += blockDim.x)
+      column_readout_one<RS>(h, rec[j], dx_inv, n, x0, y0, ybase, fa);
+    __syncthreads();   // the rows are read before the next unit overwrites them
   }
 }
 
@@ -1292,7 +1375,7 @@ void readout_columns(const float* s0, const float* s1, const float* s2, float* f
   const unsigned nstrip = (unsigned)(n / kColX) * (unsigned)(n / kColY / kColStrip);
   // algorithmic bytes: the three spectra read once, positions and forces
   const double bytes = 3.0 * (double)n * n * (M + 1) * 8.0 + (double)np * 24.0;
-  launch("readout_columns", bytes, k_column_readout<M>, dim3(nstrip), dim3(kColRows * 64), 0,
+  launch("readout_columns", bytes, k_column_readout<M>, dim3(nstrip), dim3(kColWaves * 64), 0,
          (const float2*)s0, (const float2*)s1, (const float2*)s2, (const float4*)s.rec,
          (const unsigned*)s.offsets, g, fft_twiddle_table(M), fft_twiddle_table(2 * M), f,
          fstride);

@@ -341,7 +341,7 @@ def test_tiled_readout_matches_gather_readout(fs, n_p, nc):
 @pytest.mark.timeout(600)
 def test_column_readout_matches_unfused_path(fs):
     """At 1024^3 pm.force fuses the density's tile kernel with the z r2c
-    (compute_density leaves the cells pending; tiles of 2 x 4 full z rows)
+    (compute_density leaves the cells pending; tiles of 4 x 2 full z rows)
     and the z c2r of the three force meshes with the readout (rows
     transformed in LDS, no real force mesh written).  Forces must equal, bit
     for bit, those of the path with the density written first (read back
