@@ -20,3 +20,24 @@ def pytest_configure(config):
 def fs():
     import fs2_amd.fs as fs_mod
     return fs_mod
+
+
+# The parity figures the GPU tests print ("[case] ... rel RMS ...") are
+# captured by pytest for passing tests; they are collected here and listed in
+# the terminal summary, so a quiet run (-q) still reports them.
+_FIGURES = []
+_FIGURE_KEYS = ("rel RMS", "error", "rms ", "gradient")
+
+
+def pytest_runtest_logreport(report):
+    if report.when == "call" and report.passed:
+        for line in report.capstdout.splitlines():
+            if line.startswith("[case]") and any(k in line for k in _FIGURE_KEYS):
+                _FIGURES.append(line[len("[case] "):])
+
+
+def pytest_terminal_summary(terminalreporter):
+    if _FIGURES:
+        terminalreporter.write_sep("-", "parity figures of the passing tests")
+        for line in _FIGURES:
+            terminalreporter.write_line(line)
