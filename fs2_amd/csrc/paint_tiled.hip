@@ -611,13 +611,17 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
 }
 
 // K4 fused with the z r2c of the force transform (1024^3 meshes, tiles of
-// kTileBlock / 64 full z rows): after the deterministic deposit, wavefront w
-// takes row w of the tile, forms its cell values exactly as k_tile_paint
-// stores them, transforms the row (its own int64 accumulators serve as the
-// transform's LDS scratch) and writes the half spectrum, as the z r2c pass
-// would from the stored mesh.  The real density is never written.
+// up to kTileR2cBlock / 64 full z rows): after the deterministic deposit,
+// wavefront w < rows takes row w of the tile, forms its cell values exactly
+// as k_tile_paint stores them, transforms the row (its own int64
+// accumulators serve as the transform's LDS scratch) and writes the half
+// spectrum, as the z r2c pass would from the stored mesh.  The real density
+// is never written.  The 64 KB accumulator of an 8-row tile allows two
+// workgroups per CU, so the workgroup has twice as many wavefronts as rows:
+// all of them load records and deposit, the first half transforms.
+constexpr int kTileR2cBlock = 1024;
 template <int M>
-__global__ void __launch_bounds__(kTileBlock)
+__global__ void __launch_bounds__(kTileR2cBlock)
 k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
                  TileGeom g, float fac, float fix, double fix_inv,
                  const float2* __restrict__ twM, const float2* __restrict__ twN,
@@ -644,6 +648,7 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
   tile_deposit<ACC>(acc, rec, offsets, g, tc, fac, fix, rb, rpre, &nranges);
   // row w of the tile: cells (x0 + w / ty, y0 + w % ty, all z)
   const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+  if (w >= g.tx * g.ty) return;   // no barrier follows
   const int lx = w >> g.ly, ly = w & (g.ty - 1);
   const ACC* ar = acc + (w << g.lz);
   float2 z[M / T];
@@ -1319,7 +1324,7 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
 }
 
 bool paint_r2c_supported(int n, const TileOpts& o) {
-  return n == 1024 && o.tx * o.ty * 64 == kTileBlock && o.tz == n && o.nbox == 1;
+  return n == 1024 && o.tx * o.ty * 64 <= kTileR2cBlock && o.tz == n && o.nbox == 1;
 }
 
 void paint_tiled_r2c(int64_t np, float* grid, int n, float boxsize, float fac,
@@ -1333,7 +1338,7 @@ void paint_tiled_r2c(int64_t np, float* grid, int n, float boxsize, float fac,
   // algorithmic bytes: positions read, the half spectrum written
   const double bytes = (double)np * 12.0 + (double)n * n * (M + 1) * 8.0;
   const FixedPoint fp = fixed_point(fac);
-  launch("paint_tile_r2c", bytes, k_tile_paint_r2c<M>, dim3((unsigned)g.ntiles), dim3(kTileBlock),
+  launch("paint_tile_r2c", bytes, k_tile_paint_r2c<M>, dim3((unsigned)g.ntiles), dim3(kTileR2cBlock),
          tcells * sizeof(unsigned long long), (const float4*)s.rec, (const unsigned*)s.offsets, g,
          fac, fp.fix, fp.fix_inv, fft_twiddle_table(M), fft_twiddle_table(2 * M), (float2*)grid);
 }
