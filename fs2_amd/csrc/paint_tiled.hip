@@ -881,17 +881,24 @@ __device__ __forceinline__ void column_transform_row(const float2* pa, const flo
 }
 
 // Force component at one record from the column's real rows h (slices RS
-// floats apart, ring of kColHY y rows per x row): k_tile_readout's
-// interpolation, written to the particle's id-order slot.
+// floats apart, ring of kColHY y rows per x row; the column's y0 row in
+// ring slice ring0): k_tile_readout's interpolation, written to the
+// particle's id-order slot.
 template <int RS>
 __device__ __forceinline__ void column_readout_one(const float* h, const float4 r4, float dx_inv,
-                                                   int n, int x0, int y0, int ybase, float* fa) {
+                                                   int n, int x0, int y0, int ring0, float* fa) {
+  static_assert(kColY + 1 == kColHY, "the stencil's y rows are two ring steps at most apart");
   const Cic1 cx = cic1(r4.x, dx_inv, n);
   const Cic1 cy = cic1(r4.y, dx_inv, n);
   const Cic1 cz = cic1(r4.z, dx_inv, n);
-  const int lx = cx.i0 - x0, ly = ybase + (cy.i0 - y0);
-  const int q00 = (lx * kColHY + ly % kColHY) * RS;
-  const int q01 = (lx * kColHY + (ly + 1) % kColHY) * RS;
+  const int lx = cx.i0 - x0;
+  // ring slices of the stencil's y rows y0 + dy and y0 + dy + 1, dy < kColY
+  int sa = ring0 + (cy.i0 - y0);
+  if (sa >= kColHY) sa -= kColHY;
+  int sb = sa + 1;
+  if (sb >= kColHY) sb -= kColHY;
+  const int q00 = (lx * kColHY + sa) * RS;
+  const int q01 = (lx * kColHY + sb) * RS;
   const int q10 = q00 + kColHY * RS, q11 = q01 + kColHY * RS;
   const int z0 = cz.i0, z1 = cz.i0 + 1;
   // pm.cpp:203-218: f = 0; f += (ix0 terms); f += (ix1 terms)
@@ -936,7 +943,8 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int Sy = b / nsx, Cx = b - Sy * nsx;
   const int x0 = Cx * kColX;
-  const int w0 = threadIdx.x >> 6, t0 = threadIdx.x & 63;
+  // the wavefront index is uniform: its row bookkeeping stays scalar
+  const int w0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t0 = threadIdx.x & 63;
   const float* h = (const float*)rows;
 
   float2 pa[I], pb[I];
@@ -980,10 +988,10 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
     }
     __syncthreads();
     float* fa = f + a * fstride;
-    const int ybase = kColY * c;
-    if (j0 < pend) column_readout_one<RS>(h, r0, dx_inv, n, x0, y0, ybase, fa);
+    const int ring0 = (kColY * c) % kColHY;
+    if (j0 < pend) column_readout_one<RS>(h, r0, dx_inv, n, x0, y0, ring0, fa);
     for (unsigned j = j0 + blockDim.x; j < pend; j += blockDim.x)
-      column_readout_one<RS>(h, rec[j], dx_inv, n, x0, y0, ybase, fa);
+      column_readout_one<RS>(h, rec[j], dx_inv, n, x0, y0, ring0, fa);
     __syncthreads();   // the rows are read before the next unit overwrites them
   }
 }
