@@ -37,6 +37,15 @@ struct fs2_particles {
     double m15om = 0.0;
     float kf = 0.f, q1 = 0.f, q2 = 0.f;
   } kick;
+  // A COLA drift (with the kick recorded before it, if any) waiting for the
+  // next density binning, which runs it in the same pass over the particles
+  // (paint_tiled_bin_kick_drift); any other access applies it first.  Once
+  // the device planes have been handed out, drifts run at once.
+  struct {
+    bool pending = false;
+    double dt = 0.0;
+    float da1 = 0.f, da2 = 0.f;
+  } drift;
   bool planes_exposed = false;
   float* field(int k) const { return planes + (size_t)k * 3 * np_alloc; }
   float* x() const { return field(0); }
@@ -255,11 +264,44 @@ void apply_kick(fs2_particles* p) {
             p->kick.m15om, p->kick.kf, p->kick.q1, p->kick.q2);
 }
 
+// The pending drift and the kick recorded before it as one update; both
+// are cleared.
+KickDrift take_kick_drift(fs2_particles* p) {
+  KickDrift c{};
+  c.kick = p->kick.pending ? 1 : 0;
+  c.m15om = p->kick.m15om;
+  c.kf = p->kick.kf;
+  c.q1 = p->kick.q1;
+  c.q2 = p->kick.q2;
+  c.dt = p->drift.dt;
+  c.da1 = p->drift.da1;
+  c.da2 = p->drift.da2;
+  p->kick.pending = false;
+  p->drift.pending = false;
+  return c;
+}
+
+// Runs whatever COLA update is pending: kick and drift in one pass, the
+// drift alone, or the kick alone.
+void apply_pending(fs2_particles* p) {
+  if (!p->drift.pending) {
+    apply_kick(p);
+    return;
+  }
+  const KickDrift c = take_kick_drift(p);
+  const int64_t s = (int64_t)p->np_alloc, n = (int64_t)p->np_local;
+  if (c.kick)
+    cola_kick_drift(p->x(), p->x(), p->v(), p->f(), p->dx1(), p->dx2(), s, n, c.m15om, c.kf, c.q1,
+                    c.q2, c.dt, c.da1, c.da2);
+  else
+    cola_drift(p->x(), p->v(), p->dx1(), p->dx2(), s, n, c.dt, c.da1, c.da2);
+}
+
 // Validates the handle and brings the device planes up to date: every entry
 // point that reads or writes particle data goes through here.
 fs2_particles* check_p(fs2_particles* p) {
   if (!p) fail(FS2_ERR_VALUE, "null particles handle");
-  apply_kick(p);
+  apply_pending(p);
   return p;
 }
 
@@ -695,6 +737,10 @@ int fs2_particles_device_fields(fs2_particles* p, float** x, float** v, float** 
   });
 }
 
+int fs2_particles_sync(fs2_particles* p) {
+  return guard([&] { check_p(p); });
+}
+
 int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src) {
   return guard([&] {
     if (!dst || !src) fail(FS2_ERR_VALUE, "null particles handle");
@@ -711,6 +757,7 @@ int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src) {
     dst->a_f = src->a_f;
     dst->wrap_pending = src->wrap_pending;
     dst->kick = src->kick;   // the copied planes still owe src's pending kick
+    dst->drift = src->drift; // and drift
     touch_x(dst);
   });
 }
@@ -850,14 +897,19 @@ int fs2_pm_set_paint_variant(fs2_pm* pm, int variant) {
 int fs2_pm_send_positions(fs2_pm* pm, fs2_particles* p) {
   return guard([&] {
     (void)pm;
-    check_p(p);
+    // only marks the wrap: a pending kick and drift stay pending for the
+    // density binning, which applies them before it wraps
+    if (!p) fail(FS2_ERR_VALUE, "null particles handle");
     p->wrap_pending = true;
   });
 }
 
 int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density) {
   return guard([&] {
-    check_p(p);
+    if (!p) fail(FS2_ERR_VALUE, "null particles handle");
+    // a pending COLA step runs inside the tiled paint's binning pass
+    const bool step_in_bin = p->drift.pending && pm->variant != FS2_PAINT_GLOBAL;
+    if (!step_in_bin) check_p(p);
     const int n = pm->nc;
     const float fac = pm_fac(pm);
     pm->density_pending = false;
@@ -868,7 +920,13 @@ int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density) {
     } else {
       const bool det = pm->variant == FS2_PAINT_SORTED;
       ensure_paint_scratch(pm->paint, p->np_local, paint_tiled_bins(n, pm->tiles));
-      paint_tiled_bin(pos_of(p), n, pm->boxsize_f, p->wrap_pending, pm->paint, pm->tiles);
+      if (step_in_bin) {
+        const KickDrift c = take_kick_drift(p);
+        paint_tiled_bin_kick_drift(pos_of(p), p->v(), p->f(), p->dx1(), p->dx2(), c, n,
+                                   pm->boxsize_f, p->wrap_pending, pm->paint, pm->tiles);
+      } else {
+        paint_tiled_bin(pos_of(p), n, pm->boxsize_f, p->wrap_pending, pm->paint, pm->tiles);
+      }
       // the cells wait for compute_force to fuse them with the z r2c
       pm->density_pending = det && paint_r2c_supported(n, pm->tiles);
       if (!pm->density_pending)
@@ -997,9 +1055,11 @@ int fs2_pm_power_spectrum_noalloc(fs2_pm* pm, int nbins, double dk_bin, double* 
 }
 
 // ---- COLA ---------------------------------------------------------------------------------------------------------
-// The kick is recorded and, when a drift follows, fused with it into one
-// pass over the particles (cola_kick_drift: the same two updates in order,
-// so the result is bitwise that of kick then drift).
+// The kick and the drift are recorded.  The next density binning runs both
+// in its own pass over the particles (paint_tiled_bin_kick_drift); any
+// other access first runs them as one cola_kick_drift pass.  Either way
+// the same two updates run in order, so the result is bitwise that of the
+// kick and then the drift.
 int fs2_cola_kick(fs2_particles* p, double a_vel) {
   return guard([&] {
     check_p(p);   // applies an earlier pending kick
@@ -1023,17 +1083,16 @@ int fs2_cola_drift(fs2_particles* p, double a_pos) {
     double dt, da1, da2;
     int rc = fs2_cola_drift_coefficients(p->a_x, p->a_v, a_pos, &dt, &da1, &da2);
     if (rc) throw Error{rc, g_err};
+    if (p->drift.pending) apply_pending(p);   // an earlier drift (and its kick) first
     flush_wrap(p);
-    // cola.cpp:106-109: da1, da2 stored as Float; dt stays double
-    if (p->kick.pending) {
-      p->kick.pending = false;
-      cola_kick_drift(p->x(), p->x(), p->v(), p->f(), p->dx1(), p->dx2(), (int64_t)p->np_alloc,
-                      (int64_t)p->np_local, p->kick.m15om, p->kick.kf, p->kick.q1, p->kick.q2,
-                      dt, (float)da1, (float)da2);
-    } else {
-      cola_drift(p->x(), p->v(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local,
-                 dt, (float)da1, (float)da2);
-    }
+    // cola.cpp:106-109: da1, da2 stored as Float; dt stays double.  The
+    // update runs with the next density binning (or the next other access),
+    // fused with the pending kick when there is one.
+    p->drift.dt = dt;
+    p->drift.da1 = (float)da1;
+    p->drift.da2 = (float)da2;
+    p->drift.pending = true;
+    if (p->planes_exposed) apply_pending(p);
     touch_x(p);
     p->a_x = a_pos;
   });

@@ -5,6 +5,7 @@
 // keeps the reference's mixed-precision expression order, so contraction is
 // disabled here.
 #include "fs2_internal.h"
+#include "fs2_device.h"
 
 #pragma clang fp contract(off)
 
@@ -34,15 +35,14 @@ __global__ void k_kick(float* __restrict__ v, const float* __restrict__ f,
 // sum is formed in double and rounded once into x.
 __global__ void k_drift(float* __restrict__ x, const float* __restrict__ v,
                         const float* __restrict__ dx1, const float* __restrict__ dx2,
-                        int64_t stride, int64_t np, double dt, float da1, float da2) {
+                        int64_t stride, int64_t np, KickDrift c) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int64_t j = i + a * stride;
-    const float lpt = dx1[j] * da1 + dx2[j] * da2;
-    const double rhs = (double)v[j] * dt + (double)lpt;
-    x[j] = (float)((double)x[j] + rhs);
+    float vj = v[j];
+    x[j] = kick_drift_coord(c, x[j], vj, nullptr, dx1[j], dx2[j]);
   }
 }
 
@@ -54,21 +54,15 @@ __global__ void k_drift(float* __restrict__ x, const float* __restrict__ v,
 __global__ void k_kick_drift(float* x, const float* xin, float* __restrict__ v,
                              const float* __restrict__ f, const float* __restrict__ dx1,
                              const float* __restrict__ dx2, int64_t stride, int64_t np,
-                             double m15om, float kf, float q1, float q2, double dt, float da1,
-                             float da2) {
+                             KickDrift c) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int64_t j = i + a * stride;
-    const float d1 = dx1[j], d2 = dx2[j];
-    const float s = f[j] + d1 * q1 + d2 * q2;
-    const float acc = (float)(m15om * (double)s);
-    const float vn = v[j] + acc * kf;
-    v[j] = vn;
-    const float lpt = d1 * da1 + d2 * da2;
-    const double rhs = (double)vn * dt + (double)lpt;
-    x[j] = (float)((double)xin[j] + rhs);
+    float vj = v[j];
+    x[j] = kick_drift_coord(c, xin[j], vj, f + j, dx1[j], dx2[j]);
+    v[j] = vj;
   }
 }
 
@@ -171,16 +165,22 @@ void cola_kick(float* v, const float* f, const float* dx1, const float* dx2, int
 void cola_drift(float* x, const float* v, const float* dx1, const float* dx2, int64_t stride,
                 int64_t np, double dt, float da1, float da2) {
   if (np == 0) return;
+  KickDrift c{};
+  c.kick = 0;
+  c.dt = dt;
+  c.da1 = da1;
+  c.da2 = da2;
   launch("cola_drift", (double)np * 60.0, k_drift, dim3(blocks_for(np)), dim3(kBlock), 0, x, v,
-         dx1, dx2, stride, np, dt, da1, da2);
+         dx1, dx2, stride, np, c);
 }
 
 void cola_kick_drift(float* x, const float* xin, float* v, const float* f, const float* dx1,
                      const float* dx2, int64_t stride, int64_t np, double m15om, float kf,
                      float q1, float q2, double dt, float da1, float da2) {
   if (np == 0) return;
+  const KickDrift c{1, m15om, kf, q1, q2, dt, da1, da2};
   launch("cola_kick_drift", (double)np * 84.0, k_kick_drift, dim3(blocks_for(np)), dim3(kBlock),
-         0, x, xin, v, f, dx1, dx2, stride, np, m15om, kf, q1, q2, dt, da1, da2);
+         0, x, xin, v, f, dx1, dx2, stride, np, c);
 }
 
 void gather_u32(uint32_t* dst, const uint32_t* src, const int32_t* perm, int64_t n) {

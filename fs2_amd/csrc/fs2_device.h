@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fs2_internal.h"
+
 namespace fs2 {
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the MI355X; this
@@ -23,6 +25,26 @@ __device__ __forceinline__ int xcd_remap(unsigned bid, unsigned nblocks) {
 // operator too.
 __device__ __forceinline__ float green_f2(float f1, float k2, float ka) {
   return f1 * __builtin_amdgcn_rcpf(k2) * ka;
+}
+
+// One coordinate of the COLA kick (cola.cpp:79-88) followed by the drift
+// (cola.cpp:117-124), in the reference's mixed-precision order: the kick's
+// product with -1.5 Omega_m formed in double and rounded to float, v
+// updated in float, and the drift's increment formed in double and rounded
+// once into x.  Contraction is off, so every kernel that uses it rounds the
+// same way.  kick = 0 runs the drift alone (f is not read then).
+// KickDrift is declared in fs2_internal.h.
+__device__ __forceinline__ float kick_drift_coord(const KickDrift& c, float x, float& v,
+                                                  const float* f, float d1, float d2) {
+#pragma clang fp contract(off)
+  if (c.kick) {
+    const float s = *f + d1 * c.q1 + d2 * c.q2;
+    const float acc = (float)(c.m15om * (double)s);
+    v = v + acc * c.kf;
+  }
+  const float lpt = d1 * c.da1 + d2 * c.da2;
+  const double rhs = (double)v * c.dt + (double)lpt;
+  return (float)((double)x + rhs);
 }
 
 }  // namespace fs2

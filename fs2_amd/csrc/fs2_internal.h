@@ -227,6 +227,23 @@ void pk_scratch_init();
 void fft_prepare(int n);
 
 // ---- COLA kernels (cola_kernels.hip) --------------------------------------------
+// Coefficients of a COLA kick (kick = 1; cola.cpp:57-66, Float factors and
+// -1.5 Omega_m in double) followed by a drift (cola.cpp:106-109, dt double,
+// da1 and da2 Float); kick = 0 is the drift alone.
+struct KickDrift {
+  int kick;
+  double m15om;
+  float kf, q1, q2;
+  double dt;
+  float da1, da2;
+};
+// paint_tiled_bin with the COLA step before it in the same pass over the
+// particles (paint_tiled.hip): kick (kd.kick) and drift of p.x / v as
+// cola_kick_drift or cola_drift would apply them, the wrap, then the bin
+// counts, scan and scatter of the updated positions.
+void paint_tiled_bin_kick_drift(PosArgs p, float* v, const float* f, const float* dx1,
+                                const float* dx2, const KickDrift& kd, int n, float boxsize,
+                                bool wrap, const PaintScratch& s, const TileOpts& o);
 void cola_kick(float* v, const float* f, const float* dx1, const float* dx2,
                int64_t stride, int64_t np, double m15om, float kf, float q1, float q2);
 void cola_drift(float* x, const float* v, const float* dx1, const float* dx2,

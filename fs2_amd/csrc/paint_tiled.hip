@@ -257,6 +257,54 @@ k_bin(float* __restrict__ x, int64_t stride, int64_t np, float L, int do_wrap, T
   }
 }
 
+// K1 with the COLA step that precedes it (fs2_cola_drift defers its update
+// to the next density binning): each particle's kick, if one is pending, and
+// drift as k_kick_drift / k_drift compute them (kick_drift_coord), the
+// periodic wrap, one store of the positions, and k_bin's count.  k_bin's
+// read of the positions is saved; the velocities are written only when a
+// kick ran.
+__global__ void __launch_bounds__(kBinBlock)
+k_kick_drift_bin(float* __restrict__ x, float* __restrict__ v, const float* __restrict__ f,
+                 const float* __restrict__ dx1, const float* __restrict__ dx2, int64_t stride,
+                 int64_t np, KickDrift kd, float L, int do_wrap, TileGeom g,
+                 unsigned* __restrict__ counts) {
+  __shared__ unsigned tkey[kBinSlots], tcnt[kBinSlots];
+  for (int e = threadIdx.x; e < kBinSlots; e += blockDim.x) {
+    tkey[e] = kNoKey;
+    tcnt[e] = 0u;
+  }
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * (kBinBlock * kBinPer) + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    const int64_t i = i0 + k * kBinBlock;
+    unsigned key = kNoKey;
+    if (i < np) {
+      float p[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int64_t j = i + a * stride;
+        float vj = v[j];
+        float xj = kick_drift_coord(kd, x[j], vj, f + j, dx1[j], dx2[j]);
+        if (kd.kick) v[j] = vj;
+        if (do_wrap) xj = wrap1(xj, L);
+        x[j] = xj;
+        p[a] = xj;
+      }
+      key = bin_key(g, p[0], p[1], p[2], i);
+    }
+    int start, len;
+    key_runs(key, &start, &len);
+    if (start == __lane_id() && key != kNoKey)
+      atomicAdd(&tcnt[bin_table_slot(tkey, key)], (unsigned)len);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kBinSlots; e += blockDim.x) {
+    const unsigned c = tcnt[e];
+    if (c) atomicAdd(counts + tkey[e], c);
+  }
+}
+
 // Exclusive scan of the bin counts in three launches: per-block scans of
 // kScanChunk values, a one-block scan of the block totals, and the add-back
 // that writes offsets and cursors.
@@ -1264,17 +1312,10 @@ size_t paint_tiled_bins(int n, const TileOpts& o) {
   return (size_t)g.nbox * g.bins_box;
 }
 
-void paint_tiled_bin(PosArgs p, int n, float boxsize, bool wrap, const PaintScratch& s,
-                     const TileOpts& o) {
-  const TileGeom g = make_geom(n, boxsize, o, p.np);
-  check_scratch(g, p.np, s);
+namespace {
+// K2 + K3 after the counts: the scan into offsets and cursors, the scatter.
+void bin_scan_scatter(PosArgs p, const TileGeom& g, const PaintScratch& s) {
   const int64_t nbins = (int64_t)g.nbox * g.bins_box;
-  launch("paint_clear_bins", (double)nbins * 4.0, k_fill_u32, dim3(blocks_for(nbins)),
-         dim3(kBlock), 0, s.counts, nbins, 0u);
-  if (p.np > 0) {
-    launch("paint_bin", (double)p.np * 12.0, k_bin, dim3(bin_blocks(p.np)), dim3(kBinBlock), 0, p.x,
-           p.stride, p.np, boxsize, wrap ? 1 : 0, g, s.counts);
-  }
   const int nb = (int)((nbins + kScanChunk - 1) / kScanChunk);
   if (nb > kScanChunk) fail(FS2_ERR_VALUE, "too many paint bins (%lld)", (long long)nbins);
   // cursor holds the per-block partial scan until k_scan_add rewrites it
@@ -1289,6 +1330,41 @@ void paint_tiled_bin(PosArgs p, int n, float boxsize, bool wrap, const PaintScra
     launch("paint_scatter", (double)p.np * 28.0, k_scatter, dim3(bin_blocks(p.np)),
            dim3(kBinBlock), 0, (const float*)p.x, p.stride, p.np, g, s.cursor, s.rec);
   }
+}
+
+void clear_bins(const TileGeom& g, const PaintScratch& s) {
+  const int64_t nbins = (int64_t)g.nbox * g.bins_box;
+  launch("paint_clear_bins", (double)nbins * 4.0, k_fill_u32, dim3(blocks_for(nbins)),
+         dim3(kBlock), 0, s.counts, nbins, 0u);
+}
+}  // namespace
+
+void paint_tiled_bin(PosArgs p, int n, float boxsize, bool wrap, const PaintScratch& s,
+                     const TileOpts& o) {
+  const TileGeom g = make_geom(n, boxsize, o, p.np);
+  check_scratch(g, p.np, s);
+  clear_bins(g, s);
+  if (p.np > 0) {
+    launch("paint_bin", (double)p.np * 12.0, k_bin, dim3(bin_blocks(p.np)), dim3(kBinBlock), 0, p.x,
+           p.stride, p.np, boxsize, wrap ? 1 : 0, g, s.counts);
+  }
+  bin_scan_scatter(p, g, s);
+}
+
+void paint_tiled_bin_kick_drift(PosArgs p, float* v, const float* f, const float* dx1,
+                                const float* dx2, const KickDrift& kd, int n, float boxsize,
+                                bool wrap, const PaintScratch& s, const TileOpts& o) {
+  const TileGeom g = make_geom(n, boxsize, o, p.np);
+  check_scratch(g, p.np, s);
+  clear_bins(g, s);
+  if (p.np > 0) {
+    // the kick + drift pass's bytes (84 or 60 per particle); the binning
+    // adds no traffic of its own
+    launch("cola_kick_drift_bin", (double)p.np * (kd.kick ? 84.0 : 60.0), k_kick_drift_bin,
+           dim3(bin_blocks(p.np)), dim3(kBinBlock), 0, p.x, v, f, dx1, dx2, p.stride, p.np, kd,
+           boxsize, wrap ? 1 : 0, g, s.counts);
+  }
+  bin_scan_scatter(p, g, s);
 }
 
 namespace {

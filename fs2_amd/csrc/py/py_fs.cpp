@@ -584,6 +584,15 @@ PyObject* py_particles_copy(PyObject*, PyObject* args) {
   Py_RETURN_NONE;
 }
 
+PyObject* py_particles_sync(PyObject*, PyObject* args) {
+  PyObject* o;
+  if (!PyArg_ParseTuple(args, "O", &o)) return nullptr;
+  fs2_particles* p = as_particles(o);
+  PY_ASSERT_PTR(p);
+  CALL(fs2_particles_sync(p));
+  Py_RETURN_NONE;
+}
+
 PyObject* py_particles_device_fields(PyObject*, PyObject* args) {
   PyObject* o;
   if (!PyArg_ParseTuple(args, "O", &o)) return nullptr;
@@ -1194,6 +1203,7 @@ PyMethodDef methods[] = {
     M("_particles_times", py_particles_times, "new: (a_x, a_v, a_f)"),
     M("_particles_set_times", py_particles_set_times, "new: set (a_x, a_v, a_f)"),
     M("_particles_copy", py_particles_copy, "new: device copy dst <- src"),
+    M("_particles_sync", py_particles_sync, "new: apply the deferred COLA kick and drift"),
     M("_particles_device_fields", py_particles_device_fields, "new: device pointers"),
     M("_lpt", py_lpt, "_lpt(nc, boxsize, a, seed, _ps, kind); 2LPT particles"),
     M("_lpt_set_offset", py_lpt_set_offset, "_lpt_set_offset(offset)"),

@@ -179,14 +179,21 @@ int fs2_particles_get_times(const fs2_particles* p, double* a_x, double* a_v,
  * py/cola.cpp assign them. */
 int fs2_particles_set_times(fs2_particles* p, double a_x, double a_v, double a_f);
 /* Device planes [3][stride] of each field and the id array (no reference
- * counterpart; for callers that launch their own work, e.g. autograd).  Applies pending wraps and
- * kicks; from then on fs2_cola_kick on these particles runs at once. */
+ * counterpart; for callers that launch their own work, e.g. autograd).  Applies pending wraps,
+ * kicks and drifts; from then on fs2_cola_kick and fs2_cola_drift on these particles run at
+ * once. */
 int fs2_particles_device_fields(fs2_particles* p, float** x, float** v,
                                 float** dx1, float** dx2, float** force,
                                 uint64_t** id, uint64_t* stride);
 /* Device-to-device copy of all state of src into dst (same np_alloc; no
  * reference counterpart). */
 int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src);
+/* Runs the COLA kick and drift that fs2_cola_kick / fs2_cola_drift recorded
+ * and that no later call has applied yet (no reference counterpart: the
+ * reference updates the particles inside cola.cpp:48-127 at once; here the
+ * next density binning applies them in its own pass).  For timing loops
+ * that must end with the particles' last update done. */
+int fs2_particles_sync(fs2_particles* p);
 
 /* ---- FFT grids ---------------------------------------------------------- */
 /* py/py_fft.cpp:10-30 _fft_alloc(nc): owning padded real/complex grid */

@@ -104,17 +104,21 @@ def test_kick_drift_match_oracle(fs, cosmo):
     assert p.times[0] == 0.3
 
 
-def test_deferred_kick_is_bitwise_kick_then_drift(fs, cosmo, eh):
-    """fs.cola.kick defers its velocity update into the next drift (one fused
-    pass).  Whatever touches the particles in between (reading v, another
-    pm.force, a copy) applies the kick first; every order ends bitwise equal."""
-    _announce("deferred kick: fused vs separate passes")
+def test_deferred_kick_and_drift_are_bitwise_kick_then_drift(fs, cosmo, eh):
+    """fs.cola.kick and fs.cola.drift defer their updates: the next
+    pm.force runs the kick and the drift inside its binning pass.  Whatever
+    touches the particles in between (reading v or x, another pm.force, a
+    copy, an explicit sync) applies them first, as one kick + drift pass or
+    separately; every order ends bitwise equal."""
+    _announce("deferred kick and drift: fused into the binning vs separate passes")
     n_p, nc, L = 16, 32, 64.0
     fs.pm.init(nc, nc / n_p, L)
     fs.pm.set_paint_variant("sorted")
     out = {}
-    for mode in ("fused", "read_v", "force_between", "copy"):
+    for mode in ("fused", "read_v", "read_x", "sync", "force_between", "copy", "fields"):
         p = fs.lpt.init(n_p, L, 0.1, eh, 5, "cola")
+        if mode == "fields":
+            fs._fs._particles_device_fields(p._particles)   # updates run at once from here
         for a_vel, a_pos in ocola.schedule(0.1, 1.0, 4):
             fs.pm.force(p)
             fs.cola.kick(p, a_vel)
@@ -126,11 +130,15 @@ def test_deferred_kick_is_bitwise_kick_then_drift(fs, cosmo, eh):
                 q = fs.Particles(len(p), L)
                 q.copy_from(p)      # q owes p's pending kick
                 fs.cola.drift(q, a_pos)
-                p.copy_from(q)
+                p.copy_from(q)      # and p owes q's pending kick and drift
                 continue
             fs.cola.drift(p, a_pos)
+            if mode == "read_x":
+                _ = p.x
+            elif mode == "sync":
+                fs._fs._particles_sync(p._particles)
         out[mode] = (p.x, p.v)
-    for mode in ("read_v", "force_between", "copy"):
+    for mode in out:
         for k in range(2):
             assert np.array_equal(out[mode][k], out["fused"][k]), (mode, k)
 
