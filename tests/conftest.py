@@ -1,4 +1,5 @@
 import os
+import re
 import sys
 
 import pytest
@@ -27,12 +28,14 @@ def fs():
 # the terminal summary, so a quiet run (-q) still reports them.
 _FIGURES = []
 _FIGURE_KEYS = ("rel RMS", "error", "rms ", "gradient")
+_NUMBER = re.compile(r"\d\.\d")
 
 
 def pytest_runtest_logreport(report):
     if report.when == "call" and report.passed:
         for line in report.capstdout.splitlines():
-            if line.startswith("[case]") and any(k in line for k in _FIGURE_KEYS):
+            if (line.startswith("[case]") and any(k in line for k in _FIGURE_KEYS)
+                    and _NUMBER.search(line)):
                 _FIGURES.append(line[len("[case] "):])
 
 
