@@ -264,9 +264,8 @@ void apply_kick(fs2_particles* p) {
             p->kick.m15om, p->kick.kf, p->kick.q1, p->kick.q2);
 }
 
-// The pending drift and the kick recorded before it as one update; both
-// are cleared.
-KickDrift take_kick_drift(fs2_particles* p) {
+// The pending drift and the kick recorded before it as one update.
+KickDrift pending_kick_drift(const fs2_particles* p) {
   KickDrift c{};
   c.kick = p->kick.pending ? 1 : 0;
   c.m15om = p->kick.m15om;
@@ -276,9 +275,13 @@ KickDrift take_kick_drift(fs2_particles* p) {
   c.dt = p->drift.dt;
   c.da1 = p->drift.da1;
   c.da2 = p->drift.da2;
+  return c;
+}
+
+// Called once the pending update has been launched.
+void clear_pending(fs2_particles* p) {
   p->kick.pending = false;
   p->drift.pending = false;
-  return c;
 }
 
 // Runs whatever COLA update is pending: kick and drift in one pass, the
@@ -288,13 +291,14 @@ void apply_pending(fs2_particles* p) {
     apply_kick(p);
     return;
   }
-  const KickDrift c = take_kick_drift(p);
+  const KickDrift c = pending_kick_drift(p);
   const int64_t s = (int64_t)p->np_alloc, n = (int64_t)p->np_local;
   if (c.kick)
     cola_kick_drift(p->x(), p->x(), p->v(), p->f(), p->dx1(), p->dx2(), s, n, c.m15om, c.kf, c.q1,
                     c.q2, c.dt, c.da1, c.da2);
   else
     cola_drift(p->x(), p->v(), p->dx1(), p->dx2(), s, n, c.dt, c.da1, c.da2);
+  clear_pending(p);
 }
 
 // Validates the handle and brings the device planes up to date: every entry
@@ -921,9 +925,10 @@ int fs2_pm_compute_density(fs2_pm* pm, fs2_particles* p, fs2_grid** density) {
       const bool det = pm->variant == FS2_PAINT_SORTED;
       ensure_paint_scratch(pm->paint, p->np_local, paint_tiled_bins(n, pm->tiles));
       if (step_in_bin) {
-        const KickDrift c = take_kick_drift(p);
-        paint_tiled_bin_kick_drift(pos_of(p), p->v(), p->f(), p->dx1(), p->dx2(), c, n,
-                                   pm->boxsize_f, p->wrap_pending, pm->paint, pm->tiles);
+        paint_tiled_bin_kick_drift(pos_of(p), p->v(), p->f(), p->dx1(), p->dx2(),
+                                   pending_kick_drift(p), n, pm->boxsize_f, p->wrap_pending,
+                                   pm->paint, pm->tiles);
+        clear_pending(p);
       } else {
         paint_tiled_bin(pos_of(p), n, pm->boxsize_f, p->wrap_pending, pm->paint, pm->tiles);
       }
