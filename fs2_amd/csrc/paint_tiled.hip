@@ -457,26 +457,38 @@ k_scatter(const float* __restrict__ x, int64_t stride, int64_t np, TileGeom g,
 // - rpre[r]), rpre is their exclusive prefix sum and rpre[nranges] the
 // total.  rb needs 64 entries, rpre 65.  Block-cooperative; ends with a
 // barrier.
+// Range of lane (d, cl) = (lane / 8, lane % 8) of tile_source_ranges: the
+// record range [b, b + len) of source tile T - d starting at class cl when a
+// run of selected classes starts there, else len = 0.
+__device__ __forceinline__ void source_range_lane(const unsigned* __restrict__ offsets,
+                                                  const TileGeom& g, int Tx, int Ty, int Tz,
+                                                  int lane, unsigned* b, unsigned* len) {
+  const int d = lane >> 3, cl = lane & 7;
+  const int single = (g.ntx == 1 ? 4 : 0) | (g.nty == 1 ? 2 : 0) | (g.ntz == 1 ? 1 : 0);
+  auto sel = [d](int c) { return (c & d) == d; };
+  *b = 0;
+  *len = 0;
+  if ((d & single) == 0 && sel(cl) && (cl == 0 || !sel(cl - 1))) {
+    int end = cl;
+    while (end < 7 && sel(end + 1)) ++end;
+    const unsigned Sx = (d & 4) ? (Tx ? Tx - 1 : g.ntx - 1) : Tx;
+    const unsigned Sy = (d & 2) ? (Ty ? Ty - 1 : g.nty - 1) : Ty;
+    const unsigned Sz = (d & 1) ? (Tz ? Tz - 1 : g.ntz - 1) : Tz;
+    const unsigned S8 = ((Sx * (unsigned)g.nty + Sy) * (unsigned)g.ntz + Sz) * 8u;
+    const unsigned bb = offsets[S8 + cl];
+    const unsigned e = offsets[S8 + end + 1];
+    *b = bb;
+    *len = e > bb ? e - bb : 0u;
+  }
+}
+
 __device__ __forceinline__ void tile_source_ranges(const unsigned* __restrict__ offsets,
                                                    const TileGeom& g, int Tx, int Ty, int Tz,
                                                    unsigned* rb, unsigned* rpre, int* nranges) {
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
-    const int d = lane >> 3, cl = lane & 7;
-    const int single = (g.ntx == 1 ? 4 : 0) | (g.nty == 1 ? 2 : 0) | (g.ntz == 1 ? 1 : 0);
-    auto sel = [d](int c) { return (c & d) == d; };
-    unsigned b = 0, len = 0;
-    if ((d & single) == 0 && sel(cl) && (cl == 0 || !sel(cl - 1))) {
-      int end = cl;
-      while (end < 7 && sel(end + 1)) ++end;
-      const unsigned Sx = (d & 4) ? (Tx ? Tx - 1 : g.ntx - 1) : Tx;
-      const unsigned Sy = (d & 2) ? (Ty ? Ty - 1 : g.nty - 1) : Ty;
-      const unsigned Sz = (d & 1) ? (Tz ? Tz - 1 : g.ntz - 1) : Tz;
-      const unsigned S8 = ((Sx * (unsigned)g.nty + Sy) * (unsigned)g.ntz + Sz) * 8u;
-      b = offsets[S8 + cl];
-      const unsigned e = offsets[S8 + end + 1];
-      len = e > b ? e - b : 0u;
-    }
+    unsigned b, len;
+    source_range_lane(offsets, g, Tx, Ty, Tz, lane, &b, &len);
     unsigned incl = len;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -551,6 +563,44 @@ __device__ __forceinline__ void store_group(float* p, const float* v) {
 // of its lower neighbours (tile_source_ranges; rb, rpre, nranges are its
 // shared arrays).  ACC is float (LDS float atomics) or unsigned long long
 // (int64 fixed point, deterministic).  Ends with a barrier.
+// Adds the CIC corners of one record that land in the tile with origin
+// cell (x0, y0, z0) to acc (f0 = fac, times the fixed-point scale for the
+// int64 accumulator).
+template <typename ACC>
+__device__ __forceinline__ void deposit_record(ACC* acc, const float4 r4, const TileGeom& g,
+                                               int x0, int y0, int z0, float f0) {
+  const Cic1 cx = cic1(r4.x, g.dx_inv, g.n);
+  const Cic1 cy = cic1(r4.y, g.dx_inv, g.n);
+  const Cic1 cz = cic1(r4.z, g.dx_inv, g.n);
+  const int lx[2] = {cx.i0 - x0, cx.i1 - x0};
+  const int ly[2] = {cy.i0 - y0, cy.i1 - y0};
+  const int lz[2] = {cz.i0 - z0, cz.i1 - z0};
+  const float wx[2] = {cx.w0, cx.w1};
+  const float wy[2] = {cy.w0, cy.w1};
+  const float wz[2] = {cz.w0, cz.w1};
+#pragma unroll
+  for (int ex = 0; ex < 2; ++ex) {
+    if ((unsigned)lx[ex] >= (unsigned)g.tx) continue;
+#pragma unroll
+    for (int ey = 0; ey < 2; ++ey) {
+      if ((unsigned)ly[ey] >= (unsigned)g.ty) continue;
+#pragma unroll
+      for (int ez = 0; ez < 2; ++ez) {
+        if ((unsigned)lz[ez] >= (unsigned)g.tz) continue;
+        // fac*wx*wy*wz, left to right (pm.cpp:122-132)
+        const float w = f0 * wx[ex] * wy[ey] * wz[ez];
+        const int li = (((lx[ex] << g.ly) + ly[ey]) << g.lz) + lz[ez];
+        if constexpr (sizeof(ACC) == 4) {
+          atomicAdd(&acc[li], w);
+        } else {
+          // w <= fac fix <= 2^30: one rounding to int32 per term
+          atomicAdd(&acc[li], (ACC)(long long)__float2int_rn(w));
+        }
+      }
+    }
+  }
+}
+
 template <typename ACC>
 __device__ __forceinline__ void tile_deposit(ACC* acc, const float4* __restrict__ rec,
                                              const unsigned* __restrict__ offsets,
@@ -564,38 +614,7 @@ __device__ __forceinline__ void tile_deposit(ACC* acc, const float4* __restrict_
   // fixed point: fix is a power of two, so (fac fix) wx wy wz rounds exactly
   // as (fac wx wy wz) fix
   const float f0 = sizeof(ACC) == 4 ? fac : fac * fix;
-  auto deposit = [&](const float4 r4) {
-    const Cic1 cx = cic1(r4.x, g.dx_inv, g.n);
-    const Cic1 cy = cic1(r4.y, g.dx_inv, g.n);
-    const Cic1 cz = cic1(r4.z, g.dx_inv, g.n);
-    const int lx[2] = {cx.i0 - x0, cx.i1 - x0};
-    const int ly[2] = {cy.i0 - y0, cy.i1 - y0};
-    const int lz[2] = {cz.i0 - z0, cz.i1 - z0};
-    const float wx[2] = {cx.w0, cx.w1};
-    const float wy[2] = {cy.w0, cy.w1};
-    const float wz[2] = {cz.w0, cz.w1};
-#pragma unroll
-    for (int ex = 0; ex < 2; ++ex) {
-      if ((unsigned)lx[ex] >= (unsigned)g.tx) continue;
-#pragma unroll
-      for (int ey = 0; ey < 2; ++ey) {
-        if ((unsigned)ly[ey] >= (unsigned)g.ty) continue;
-#pragma unroll
-        for (int ez = 0; ez < 2; ++ez) {
-          if ((unsigned)lz[ez] >= (unsigned)g.tz) continue;
-          // fac*wx*wy*wz, left to right (pm.cpp:122-132)
-          const float w = f0 * wx[ex] * wy[ey] * wz[ez];
-          const int li = (((lx[ex] << g.ly) + ly[ey]) << g.lz) + lz[ez];
-          if constexpr (sizeof(ACC) == 4) {
-            atomicAdd(&acc[li], w);
-          } else {
-            // w <= fac fix <= 2^30: one rounding to int32 per term
-            atomicAdd(&acc[li], (ACC)(long long)__float2int_rn(w));
-          }
-        }
-      }
-    }
-  };
+  auto deposit = [&](const float4 r4) { deposit_record<ACC>(acc, r4, g, x0, y0, z0, f0); };
   // two records per thread per round, both loads issued before either is
   // deposited (a tile holds ~1.3 records per thread at 1 particle per 8
   // cells, so most threads need one round)
@@ -659,17 +678,88 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
 }
 
 // K4 fused with the z r2c of the force transform (1024^3 meshes, tiles of
-// up to kTileR2cBlock / 64 full z rows): after the deterministic deposit,
+// at most kR2cRowWaves full z rows): after the deterministic deposit,
 // wavefront w < rows takes row w of the tile, forms its cell values exactly
 // as k_tile_paint stores them, transforms the row (its own int64
 // accumulators serve as the transform's LDS scratch) and writes the half
 // spectrum, as the z r2c pass would from the stored mesh.  The real density
-// is never written.  The 64 KB accumulator of an 8-row tile allows two
-// workgroups per CU, so the workgroup has twice as many wavefronts as rows:
-// all of them load records and deposit, the first half transforms.
+// is never written.
+//
+// A tile's deposit waits on two dependent loads (its bin offsets, then its
+// records) and the 64 KB accumulator of an 8-row tile leaves room for only
+// two workgroups per CU, so each workgroup walks kR2cTiles consecutive
+// tiles and pipelines them: while wavefronts 0-7 transform and write tile
+// k (then clear their rows), wavefronts 8-15 find tile k + 1's source
+// ranges and load up to kR2cPre records each into registers.  The next
+// deposit starts with its records in hand; records beyond the prefetch are
+// read from the range table wavefront 8 leaves in LDS.  At 512^3 particles
+// on the clustered run: 2.90 ms per launch unpipelined; 2.53, 2.42-2.44,
+// 2.37 and 2.43 ms with 4, 8, 16 and 32 tiles per workgroup (3 records
+// prefetched per thread; 2 records: 2.46-2.48 at 8 tiles).
 constexpr int kTileR2cBlock = 1024;
+constexpr int kR2cRowWaves = 8;
+constexpr unsigned kR2cLoaders = kTileR2cBlock - kR2cRowWaves * 64;
+constexpr int kR2cPre = 3;
+constexpr int kR2cTiles = 16;
+
+// Source ranges and prefetched records of tile tc, for one loader thread
+// (index lt < kR2cLoaders, lane `lane` of its wavefront).  Every loader
+// wavefront forms the tile's ranges itself (source_range_lane and a
+// wavefront scan; a record's range is found with scalar reads of the
+// range starts); `table` wavefronts also write them to rb / rpre / nranges
+// as tile_source_ranges does.  Returns the tile's record count.
+__device__ __forceinline__ unsigned r2c_prefetch(const float4* __restrict__ rec,
+                                                 const unsigned* __restrict__ offsets,
+                                                 const TileGeom& g, const TileCoord& tc,
+                                                 unsigned lt, int lane, bool table,
+                                                 unsigned* rb, unsigned* rpre, int* nranges,
+                                                 float4* pre) {
+  unsigned b, len;
+  source_range_lane(offsets, g, tc.Tx, tc.Ty, tc.Tz, lane, &b, &len);
+  unsigned incl = len;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned y = __shfl_up(incl, off);
+    if (lane >= off) incl += y;
+  }
+  const unsigned excl = incl - len;
+  const uint64_t nonempty = __ballot(len > 0);
+  const unsigned total = (unsigned)__shfl((int)incl, 63);
+  if (table) {
+    const int k = __popcll(nonempty & ((1ull << lane) - 1ull));
+    if (len > 0) {
+      rb[k] = b;
+      rpre[k] = excl;
+    }
+    if (lane == 63) {
+      *nranges = __popcll(nonempty);
+      rpre[__popcll(nonempty)] = incl;
+    }
+  }
+  unsigned src[kR2cPre];
+#pragma unroll
+  for (int i = 0; i < kR2cPre; ++i) src[i] = 0u;
+  // the ranges in order: record r lies in the last range whose start <= r
+  for (uint64_t m = nonempty; m; m &= m - 1ull) {
+    const int sl = __builtin_ctzll(m);
+    const unsigned p0 = (unsigned)__builtin_amdgcn_readlane((int)excl, sl);
+    const unsigned b0 = (unsigned)__builtin_amdgcn_readlane((int)b, sl);
+#pragma unroll
+    for (int i = 0; i < kR2cPre; ++i) {
+      const unsigned r = lt + (unsigned)i * kR2cLoaders;
+      if (r >= p0) src[i] = b0 + (r - p0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kR2cPre; ++i) {
+    const unsigned r = lt + (unsigned)i * kR2cLoaders;
+    if (r < total) pre[i] = rec[src[i]];
+  }
+  return total;
+}
+
 template <int M>
-__global__ void __launch_bounds__(kTileR2cBlock)
+__global__ void __launch_bounds__(kTileR2cBlock) __attribute__((amdgpu_waves_per_eu(8)))
 k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ offsets,
                  TileGeom g, float fac, float fix, double fix_inv,
                  const float2* __restrict__ twM, const float2* __restrict__ twN,
@@ -684,42 +774,91 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
   ACC* acc = (ACC*)smem;
   __shared__ float2 stM[M];
   __shared__ float2 stN[M];
+  __shared__ unsigned rb[64], rpre[65];
+  __shared__ int nranges;
   for (int i = threadIdx.x; i < M; i += blockDim.x) {
     stM[i] = twM[i];
     stN[i] = twN[i];
   }
   const int tcells = g.tx * g.ty * g.tz;   // tz = n = 2M: tx * ty rows
+  const int rows = g.tx * g.ty;
   lds_zero(smem, tcells * (int)sizeof(ACC));
-  const TileCoord tc = tile_coord(g, xcd_remap(blockIdx.x, gridDim.x));
-  __shared__ unsigned rb[64], rpre[65];
-  __shared__ int nranges;
-  tile_deposit<ACC>(acc, rec, offsets, g, tc, fac, fix, rb, rpre, &nranges);
-  // row w of the tile: cells (x0 + w / ty, y0 + w % ty, all z)
-  const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
-  if (w >= g.tx * g.ty) return;   // no barrier follows
-  const int lx = w >> g.ly, ly = w & (g.ty - 1);
-  const ACC* ar = acc + (w << g.lz);
-  float2 z[M / T];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const bool loader = w >= kR2cRowWaves;
+  const unsigned lt = threadIdx.x - (unsigned)(kR2cRowWaves * 64);
+  const unsigned vb = (unsigned)xcd_remap(blockIdx.x, gridDim.x) * (unsigned)kR2cTiles;
+  // fixed point: fix is a power of two, so (fac fix) wx wy wz rounds exactly
+  // as (fac wx wy wz) fix
+  const float f0 = fac * fix;
+  float4 pre[kR2cPre];
 #pragma unroll
-  for (int i = 0; i < M / T; ++i) {
-    const int m = t + i * T;
-    z[i] = make_float2(cell_value<ACC>(ar[2 * m], fix_inv), cell_value<ACC>(ar[2 * m + 1], fix_inv));
-  }
-  float2* sp = (float2*)(acc + (w << g.lz));
-  fft_regs<M, L, WaveSync>(z, sp, t, stM);   // starts with a wavefront barrier
-  WaveSync::sync();   // the row's lanes are done reading the last stage's input
+  for (int i = 0; i < kR2cPre; ++i) pre[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  unsigned total = 0;
+  if (loader)
+    total = r2c_prefetch(rec, offsets, g, tile_coord(g, vb), lt, lane, w == kR2cRowWaves, rb,
+                         rpre, &nranges, pre);
+  __syncthreads();   // accumulators clear, twiddles and the first tile's table staged
+#pragma unroll 1
+  for (int k = 0; k < kR2cTiles; ++k) {
+    const TileCoord tc = tile_coord(g, vb + (unsigned)k);
+    const int x0 = tc.Tx * g.tx, y0 = tc.Ty * g.ty, z0 = tc.Tz * g.tz;
+    if (loader) {
 #pragma unroll
-  for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = z[i];
-  WaveSync::sync();
-  float2* r = grid + ((int64_t)(tc.Tx * g.tx + lx) * g.n + (tc.Ty * g.ty + ly)) * (g.ncz / 2);
+      for (int i = 0; i < kR2cPre; ++i)
+        if (lt + (unsigned)i * kR2cLoaders < total)
+          deposit_record<ACC>(acc, pre[i], g, x0, y0, z0, f0);
+    }
+    // records beyond the prefetch, located through the LDS range table
+    const unsigned tot = rpre[nranges];
+    int r = 0;
+    for (unsigned v = kR2cLoaders * kR2cPre + threadIdx.x; v < tot; v += blockDim.x) {
+      while (v >= rpre[r + 1]) ++r;
+      deposit_record<ACC>(acc, rec[rb[r] + (v - rpre[r])], g, x0, y0, z0, f0);
+    }
+    __syncthreads();   // the tile's cells are complete
+    if (!loader) {
+      if (w < rows) {
+        // row w of the tile: cells (x0 + w / ty, y0 + w % ty, all z)
+        const int lx = w >> g.ly, ly = w & (g.ty - 1);
+        ACC* ar = acc + (w << g.lz);
+        // an opaque copy of the lane: without it the compiler keeps every
+        // LDS address of the transform live across the tile loop
+        int t = threadIdx.x & 63;
+        asm volatile("" : "+v"(t));
+        float2 z[M / T];
 #pragma unroll
-  for (int i = 0; i < M / T; ++i) {
-    const int k = t + i * T;
-    r[k] = r2c_split(sp[L::at(k)], sp[L::at(k == 0 ? 0 : M - k)], stN[k]);
-  }
-  if (t == 0) {
-    const float2 A = sp[L::at(0)];
-    r[M] = make_float2(A.x - A.y, 0.f);
+        for (int i = 0; i < M / T; ++i) {
+          const int m = t + i * T;
+          z[i] = make_float2(cell_value<ACC>(ar[2 * m], fix_inv),
+                             cell_value<ACC>(ar[2 * m + 1], fix_inv));
+        }
+        float2* sp = (float2*)ar;
+        fft_regs<M, L, WaveSync>(z, sp, t, stM);   // starts with a wavefront barrier
+        WaveSync::sync();   // the row's lanes are done reading the last stage's input
+#pragma unroll
+        for (int i = 0; i < M / T; ++i) sp[L::at(t + i * T)] = z[i];
+        WaveSync::sync();
+        float2* out = grid + ((int64_t)(x0 + lx) * g.n + (y0 + ly)) * (g.ncz / 2);
+#pragma unroll
+        for (int i = 0; i < M / T; ++i) {
+          const int kk = t + i * T;
+          out[kk] = r2c_split(sp[L::at(kk)], sp[L::at(kk == 0 ? 0 : M - kk)], stN[kk]);
+        }
+        if (t == 0) {
+          const float2 A = sp[L::at(0)];
+          out[M] = make_float2(A.x - A.y, 0.f);
+        }
+        WaveSync::sync();   // the row is read before it is cleared for the next tile
+        uint4* a4 = (uint4*)ar;
+#pragma unroll
+        for (int i = 0; i < 2 * M * (int)sizeof(ACC) / 16 / 64; ++i)
+          a4[t + i * 64] = make_uint4(0u, 0u, 0u, 0u);
+      }
+    } else if (k + 1 < kR2cTiles) {
+      total = r2c_prefetch(rec, offsets, g, tile_coord(g, vb + (unsigned)k + 1u), lt, lane,
+                           w == kR2cRowWaves, rb, rpre, &nranges, pre);
+    }
+    __syncthreads();   // rows written and cleared, the next tile's table staged
   }
 }
 
@@ -1408,7 +1547,7 @@ void paint_tiled(PosArgs p, float* grid, int n, float boxsize, float fac, bool w
 }
 
 bool paint_r2c_supported(int n, const TileOpts& o) {
-  return n == 1024 && o.tx * o.ty * 64 <= kTileR2cBlock && o.tz == n && o.nbox == 1;
+  return n == 1024 && o.tx * o.ty <= kR2cRowWaves && o.tz == n && o.nbox == 1;
 }
 
 void paint_tiled_r2c(int64_t np, float* grid, int n, float boxsize, float fac,
@@ -1422,7 +1561,11 @@ void paint_tiled_r2c(int64_t np, float* grid, int n, float boxsize, float fac,
   // algorithmic bytes: positions read, the half spectrum written
   const double bytes = (double)np * 12.0 + (double)n * n * (M + 1) * 8.0;
   const FixedPoint fp = fixed_point(fac);
-  launch("paint_tile_r2c", bytes, k_tile_paint_r2c<M>, dim3((unsigned)g.ntiles), dim3(kTileR2cBlock),
+  if (g.ntiles % kR2cTiles)
+    fail(FS2_ERR_VALUE, "fused paint + r2c: %d tiles are not a multiple of %d", g.ntiles,
+         kR2cTiles);
+  launch("paint_tile_r2c", bytes, k_tile_paint_r2c<M>, dim3((unsigned)(g.ntiles / kR2cTiles)),
+         dim3(kTileR2cBlock),
          tcells * sizeof(unsigned long long), (const float4*)s.rec, (const unsigned*)s.offsets, g,
          fac, fp.fix, fp.fix_inv, fft_twiddle_table(M), fft_twiddle_table(2 * M), (float2*)grid);
 }
