@@ -1147,10 +1147,12 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
     const int y0 = Cy * kColY;
     const unsigned col = ((unsigned)Cx * nty + (unsigned)Cy) * ntz;
     const unsigned pbeg = offsets[col * 8u], pend = offsets[(col + ntz) * 8u];
-    // this thread's first record of the column, in flight during the transform
-    const unsigned j0 = pbeg + threadIdx.x;
-    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    // this thread's first two records of the column, in flight during the
+    // transform
+    const unsigned j0 = pbeg + threadIdx.x, j1 = j0 + blockDim.x;
+    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
     if (j0 < pend) r0 = rec[j0];
+    if (j1 < pend) r1 = rec[j1];
     const int nrows = c == 0 ? kColRows : kColNew;
     int r = w0;
     if (r < nrows) {   // the row loaded during the last unit
@@ -1177,7 +1179,8 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
     float* fa = f + a * fstride;
     const int ring0 = (kColY * c) % kColHY;
     if (j0 < pend) column_readout_one<RS>(h, r0, dx_inv, n, x0, y0, ring0, fa);
-    for (unsigned j = j0 + blockDim.x; j < pend; j += blockDim.x)
+    if (j1 < pend) column_readout_one<RS>(h, r1, dx_inv, n, x0, y0, ring0, fa);
+    for (unsigned j = j1 + blockDim.x; j < pend; j += blockDim.x)
       column_readout_one<RS>(h, rec[j], dx_inv, n, x0, y0, ring0, fa);
     __syncthreads();   // the rows are read before the next unit overwrites them
   }
