@@ -22,6 +22,12 @@
 //               fixed-point LDS sums (deterministic variant: integer addition
 //               is associative, so the result is bitwise reproducible).
 //
+// At 1024^3 the tiles are columns of 4 x 2 cells and every z cell: K4 runs
+// fused with the density's z r2c (k_tile_paint_r2c), and the readout fused
+// with the z c2r of the three force spectra (k_column_readout).  K1 also
+// exists with the COLA kick and drift that the previous step deferred in
+// front of it (k_kick_drift_bin).
+//
 // CIC indices and weights follow pm.cpp:85-133 in FP32 with contraction
 // disabled, exactly as the per-particle atomic kernel does.
 #include <algorithm>
