@@ -693,8 +693,10 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
 //
 // A tile's deposit waits on two dependent loads (its bin offsets, then its
 // records) and the 64 KB accumulator of an 8-row tile leaves room for only
-// two workgroups per CU, so each workgroup walks kR2cTiles consecutive
-// tiles and pipelines them: while wavefronts 0-7 transform and write tile
+// two workgroups per CU, so each workgroup walks kR2cTiles tiles (strided
+// by the grid, so that the tiles in flight together stay neighbours: 2.6 GB
+// read per launch against 3.8 GB for consecutive tiles, same time) and
+// pipelines them: while wavefronts 0-7 transform and write tile
 // k (then clear their rows), wavefronts 8-15 find tile k + 1's source
 // ranges and load up to kR2cPre records each into registers.  The next
 // deposit starts with its records in hand; records beyond the prefetch are
@@ -792,7 +794,10 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const bool loader = w >= kR2cRowWaves;
   const unsigned lt = threadIdx.x - (unsigned)(kR2cRowWaves * 64);
-  const unsigned vb = (unsigned)xcd_remap(blockIdx.x, gridDim.x) * (unsigned)kR2cTiles;
+  // tile k of this workgroup: k * gridDim.x + its (XCD-remapped) index, so
+  // that the tiles in flight at one time are neighbours, as in a one-shot
+  // launch, and find the spilling records of their lower neighbours in L2
+  const unsigned vb = (unsigned)xcd_remap(blockIdx.x, gridDim.x), vs = gridDim.x;
   // fixed point: fix is a power of two, so (fac fix) wx wy wz rounds exactly
   // as (fac wx wy wz) fix
   const float f0 = fac * fix;
@@ -806,7 +811,7 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
   __syncthreads();   // accumulators clear, twiddles and the first tile's table staged
 #pragma unroll 1
   for (int k = 0; k < kR2cTiles; ++k) {
-    const TileCoord tc = tile_coord(g, vb + (unsigned)k);
+    const TileCoord tc = tile_coord(g, vb + (unsigned)k * vs);
     const int x0 = tc.Tx * g.tx, y0 = tc.Ty * g.ty, z0 = tc.Tz * g.tz;
     if (loader) {
 #pragma unroll
@@ -861,7 +866,7 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
           a4[t + i * 64] = make_uint4(0u, 0u, 0u, 0u);
       }
     } else if (k + 1 < kR2cTiles) {
-      total = r2c_prefetch(rec, offsets, g, tile_coord(g, vb + (unsigned)k + 1u), lt, lane,
+      total = r2c_prefetch(rec, offsets, g, tile_coord(g, vb + (unsigned)(k + 1) * vs), lt, lane,
                            w == kR2cRowWaves, rb, rpre, &nranges, pre);
     }
     __syncthreads();   // rows written and cleared, the next tile's table staged
