@@ -112,10 +112,18 @@ void drop_graph(fs2_batch* b) {
   b->graph = nullptr;
 }
 
-// density -> force meshes -> forces of every box of the class
-void class_force(BatchClass& c) {
+// density -> force meshes -> forces of every box of the class; with `step`,
+// the previous step's kick and drift run inside the density's binning pass
+// (paint_tiled_bin_kick_drift, bitwise the same as running them first)
+void class_force(BatchClass& c, const KickDrift* step = nullptr) {
   const PosArgs pos{c.field(0), c.np, c.np};
-  paint_tiled(pos, c.mesh, c.nm, c.boxsize_f, c.fac, true, true, c.paint, c.tile);
+  if (step) {
+    paint_tiled_bin_kick_drift(pos, c.field(1), c.field(4), c.field(2), c.field(3), *step, c.nm,
+                               c.boxsize_f, true, c.paint, c.tile);
+    paint_tiled_cells(c.np, c.mesh, c.nm, c.boxsize_f, c.fac, true, c.paint, c.tile);
+  } else {
+    paint_tiled(pos, c.mesh, c.nm, c.boxsize_f, c.fac, true, true, c.paint, c.tile);
+  }
   fft_force_meshes(c.mesh, c.F, c.nm, c.f1, c.nbox);
   readout_tiled(c.F[0], c.F[1], c.F[2], c.field(4), c.np, c.np, c.nm, c.boxsize_f, c.paint,
                 c.tile);
@@ -156,8 +164,19 @@ void run_schedule(fs2_batch* b) {
     BatchClass& c = b->cls[k];
     FS2_HIP(hipStreamWaitEvent(c.s, b->fork, 0));
     StreamScope scope(c.s);
-    for (const StepCoef& s : coef) {
-      class_force(c);
+    // each step's kick and drift run in the next step's binning pass; the
+    // last step's after the loop
+    for (size_t i = 0; i < coef.size(); ++i) {
+      if (i == 0) {
+        class_force(c);
+      } else {
+        const StepCoef& s = coef[i - 1];
+        const KickDrift kd{1, m15om, s.kf, s.q1, s.q2, s.dt, s.da1, s.da2};
+        class_force(c, &kd);
+      }
+    }
+    if (!coef.empty()) {
+      const StepCoef& s = coef.back();
       cola_kick_drift(c.field(0), c.field(0), c.field(1), c.field(4), c.field(2), c.field(3), c.np,
                       c.np, m15om, s.kf, s.q1, s.q2, s.dt, s.da1, s.da2);
     }
