@@ -326,29 +326,26 @@ def copy_probe(fs, gib=4):
 # is one r2c and three Green-multiplied c2r transforms, each reading and
 # writing its mesh once (4 Ng + 8 Nk); the readout reads x and three meshes
 # and writes f; kick and drift each read and write 60 bytes per particle.
-# Where a kernel fuses work of two stages (the paint's tile kernel with the
-# z r2c, the three z c2r with the readout), the stage it is timed in also
-# carries the compulsory bytes of the transforms it contains, and the FFT
-# stage keeps what is left (nothing once both ends are fused: its y and x
-# passes then move only intermediate meshes).  The stage bytes always sum
-# to 156 Np + 32 Ng + 32 Nk.
+# Where kernels fuse the paint with the z r2c and the three z c2r with the
+# readout, a transform's compulsory bytes can no longer be set against the
+# time of its own passes (its z pass runs inside the paint or the readout,
+# its y and x passes on their own), so the paint, the FFT and the readout
+# form one force stage.  The stage bytes always sum to 156 Np + 32 Ng +
+# 32 Nk.  The bin count runs inside the kick + drift kernel after the first
+# step and is timed there.
 def stage_table(prof):
     names = {r[0] for r in prof}
-    r2c_in_paint = "paint_tile_r2c" in names
-    c2r_in_readout = "readout_columns" in names
-    xform = lambda Ng, Nk: 4 * Ng + 8 * Nk
+    fused = "paint_tile_r2c" in names or "readout_columns" in names
+    kick_drift = {"kick_drift": (("cola_",), lambda Np, Ng, Nk: 120 * Np)}
+    if fused:
+        return {"paint+fft+readout": (("paint_", "fft_", "readout"),
+                                      lambda Np, Ng, Nk: 36 * Np + 32 * Ng + 32 * Nk),
+                **kick_drift}
     return {
-        "paint+z_r2c" if r2c_in_paint else "paint": (
-            ("paint_",),
-            lambda Np, Ng, Nk: 12 * Np + 4 * Ng + (xform(Ng, Nk) if r2c_in_paint else 0)),
-        "fft": (
-            ("fft_",),
-            lambda Np, Ng, Nk: 16 * Ng + 32 * Nk - (xform(Ng, Nk) if r2c_in_paint else 0)
-            - (3 * xform(Ng, Nk) if c2r_in_readout else 0)),
-        "z_c2r+readout" if c2r_in_readout else "readout": (
-            ("readout",),
-            lambda Np, Ng, Nk: 24 * Np + 12 * Ng + (3 * xform(Ng, Nk) if c2r_in_readout else 0)),
-        "kick_drift": (("cola_",), lambda Np, Ng, Nk: 120 * Np),
+        "paint": (("paint_",), lambda Np, Ng, Nk: 12 * Np + 4 * Ng),
+        "fft": (("fft_",), lambda Np, Ng, Nk: 16 * Ng + 32 * Nk),
+        "readout": (("readout",), lambda Np, Ng, Nk: 24 * Np + 12 * Ng),
+        **kick_drift,
     }
 
 

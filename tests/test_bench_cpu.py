@@ -42,8 +42,9 @@ def test_more_than_one_gpu_is_refused():
 
 def test_stage_bytes_with_fused_kernels():
     """With the fused paint + z r2c and z c2r + readout kernels in the
-    profile, the stages keep the compulsory bytes of the work they contain
-    and still sum to 156 Np + 32 Ng + 32 Nk; the y/x passes carry none."""
+    profile, the paint, the FFT passes and the readout form one stage with
+    their compulsory bytes together; the stages still sum to 156 Np + 32 Ng
+    + 32 Nk."""
     Np, n = 512**3, 1024
     Ng, Nk = n**3, n * n * (n // 2 + 1)
     steps = 2
@@ -52,12 +53,12 @@ def test_stage_bytes_with_fused_kernels():
             ("fft_x_fwd_green3", steps, 8.0, 0.0), ("readout_columns", steps, 14.0, 0.0),
             ("cola_kick_drift", steps, 4.0, 0.0)]
     st = bench.stage_roofline(prof, steps, Np, n)
-    assert set(st) == {"paint+z_r2c", "fft", "z_c2r+readout", "kick_drift"}
+    assert set(st) == {"paint+fft+readout", "kick_drift"}
     assert sum(v["algorithmic_bytes"] for v in st.values()) == 156 * Np + 32 * Ng + 32 * Nk
-    assert st["paint+z_r2c"]["algorithmic_bytes"] == 12 * Np + 8 * Ng + 8 * Nk
-    assert st["z_c2r+readout"]["algorithmic_bytes"] == 24 * Np + 24 * Ng + 24 * Nk
-    assert st["fft"]["algorithmic_bytes"] == 0 and st["fft"]["frac"] is None
-    assert st["fft"]["ms"] == 6.0 and st["paint+z_r2c"]["ms"] == 4.5
+    assert st["paint+fft+readout"]["algorithmic_bytes"] == 36 * Np + 32 * Ng + 32 * Nk
+    assert st["paint+fft+readout"]["ms"] == 17.5 and st["kick_drift"]["ms"] == 2.0
+    frac = (36 * Np + 32 * Ng + 32 * Nk) / 17.5e-3 / 8e12
+    assert abs(st["paint+fft+readout"]["frac"] - frac) < 1e-4
     # unfused: the four stages of the reference's structure
     st = bench.stage_roofline([r for r in prof if r[0] not in ("paint_tile_r2c", "readout_columns")]
                               + [("paint_tile_det", steps, 4.0, 0.0), ("fft_z_r2c", steps, 4.0, 0.0),
