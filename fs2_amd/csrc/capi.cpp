@@ -259,9 +259,9 @@ namespace {
 
 void apply_kick(fs2_particles* p) {
   if (!p->kick.pending) return;
-  p->kick.pending = false;
   cola_kick(p->v(), p->f(), p->dx1(), p->dx2(), (int64_t)p->np_alloc, (int64_t)p->np_local,
             p->kick.m15om, p->kick.kf, p->kick.q1, p->kick.q2);
+  p->kick.pending = false;   // only once launched: a failed launch leaves it owed
 }
 
 // The pending drift and the kick recorded before it as one update.
@@ -763,6 +763,10 @@ int fs2_particles_copy(fs2_particles* dst, const fs2_particles* src) {
     dst->kick = src->kick;   // the copied planes still owe src's pending kick
     dst->drift = src->drift; // and drift
     touch_x(dst);
+    if (dst->planes_exposed) {   // planes handed out to the caller hold every update
+      apply_pending(dst);        // (a pending drift predates a pending wrap)
+      flush_wrap(dst);
+    }
   });
 }
 
@@ -902,9 +906,14 @@ int fs2_pm_send_positions(fs2_pm* pm, fs2_particles* p) {
   return guard([&] {
     (void)pm;
     // only marks the wrap: a pending kick and drift stay pending for the
-    // density binning, which applies them before it wraps
+    // density binning, which applies them before it wraps.  Planes handed
+    // out to the caller are brought up to date at once.
     if (!p) fail(FS2_ERR_VALUE, "null particles handle");
     p->wrap_pending = true;
+    if (p->planes_exposed) {
+      apply_pending(p);
+      flush_wrap(p);
+    }
   });
 }
 

@@ -168,6 +168,49 @@ def test_positions_written_through_device_pointers_are_read_out(fs, eh):
     assert rel < 1e-5, rel
 
 
+def test_exposed_planes_hold_the_wrap_and_copied_updates(fs, eh):
+    """Once fs2_particles_device_fields has handed out the planes, each
+    update is in them when its call returns: send_positions' periodic wrap
+    (pm_domain.cpp:94-126 wraps in place), and the kick and drift that a
+    copy brings along from particles whose updates are still deferred."""
+    _announce("exposed planes: wrap and copied deferred updates")
+    n_p, nc, L = 8, 16, 32.0
+    fs.pm.init(nc, nc / n_p, L)
+    hip = _hip()
+
+    def raw(fields, name, n):
+        fs._fs._synchronize()
+        out = np.empty((3, n), np.float32)
+        for c in range(3):
+            src = fields[name] + 4 * c * fields["stride"]
+            assert hip.hipMemcpy(out[c].ctypes.data, src, 4 * n, 2) == 0
+        return out.T
+
+    p = fs.lpt.init(n_p, L, 0.1, eh, 3, "cola")
+    x = p.x.copy()
+    x[::3, 0] += L
+    x[1::3, 1] -= L
+    p.set_field("x", x)
+    fields = p.device_fields()
+    assert raw(fields, "x", len(x)).max() >= L   # not yet wrapped
+    fs.pm.send_positions(p)
+    xr = raw(fields, "x", len(x))
+    assert xr.min() >= 0.0 and xr.max() < L
+    assert np.array_equal(xr, p.x)
+
+    src = fs.lpt.init(n_p, L, 0.1, eh, 3, "cola")
+    fs.pm.force(src)
+    fs.cola.kick(src, 0.15)
+    fs.cola.drift(src, 0.2)        # both deferred in src
+    q = fs.Particles(len(src), L)
+    qf = q.device_fields()
+    q.copy_from(src)
+    xq, vq = raw(qf, "x", len(src)), raw(qf, "v", len(src))
+    # src's getters apply its pending updates: the same kernels, same values
+    assert np.array_equal(xq, src.x) and np.array_equal(vq, src.v)
+    assert not np.array_equal(vq, np.zeros_like(vq))
+
+
 def test_linear_regime_keeps_the_lpt_trajectory(fs, tmp_path):
     """A tiny, smooth (k < 0.15 h/Mpc) spectrum: the PM force is close to the
     linear force -D1 dx1, so the 10 COLA steps keep x near
