@@ -382,6 +382,36 @@ def test_column_readout_matches_unfused_path(fs):
         assert np.array_equal(fs.pm.force_mesh(a)[::7, 3::11, :], mesh_col[a]), a
 
 
+@pytest.mark.timeout(300)
+def test_fused_path_with_empty_and_sparse_sets(fs):
+    """The 1024^3 fused kernels (paint + z r2c, column readout) with almost
+    every tile and column empty: no particles at all (the reference's empty
+    domain), then six, two of them on the box's upper faces.  The sparse
+    forces equal, bit for bit, those of the unfused path."""
+    _announce("fused 1024^3 path: empty and sparse particle sets")
+    nc, L = 1024, 1024.0
+    fs.pm.init(nc, 2.0, L)
+    fs.pm.set_paint_variant("sorted")
+    p = fs.Particles(16, L)
+    p.set_times(0.5, 0.5, 0.0)
+    fs.pm.force(p)
+    assert p.force.shape == (0, 3)
+    x = np.random.default_rng(23).uniform(0, L, (6, 3))
+    x[0] = [L * (1 - 1e-7), 3.5, 1000.25]     # rounds to cell n -> index 0
+    x[1] = [511.9, L - 0.01, L - 0.5]
+    p.append(x)
+    p.set_times(0.5, 0.5, 0.0)
+    fs.pm.force(p)                            # fused
+    f_fused = p.force.copy()
+    assert np.isfinite(f_fused).all() and np.abs(f_fused).max() > 0
+    p.set_times(0.5, 0.5, -1.0)
+    fs.pm.send_positions(p)
+    fs.pm.compute_density(p)
+    p.set_field("x", p.x)                     # new version -> unfused path
+    fs.pm.compute_force(p)
+    assert np.array_equal(p.force, f_fused), np.abs(p.force - f_fused).max()
+
+
 def test_mem_named_allocation(fs):
     """fs.mem.Mem (py/fs/mem.py): sizes rounded up to 16 bytes (mem.cpp:48-50),
     nothing allocated for size 0, MemoryError when the device cannot hold it."""
