@@ -463,29 +463,46 @@ k_scatter(const float* __restrict__ x, int64_t stride, int64_t np, TileGeom g,
 // - rpre[r]), rpre is their exclusive prefix sum and rpre[nranges] the
 // total.  rb needs 64 entries, rpre 65.  Block-cooperative; ends with a
 // barrier.
+
+// The two bin offsets that bound lane (d, cl)'s range (loads only, so that
+// a caller can issue other loads before it uses them; act = false: no run
+// of selected classes starts at this lane).
+struct RangeBounds {
+  unsigned bb = 0, e = 0;
+  bool act = false;
+};
+__device__ __forceinline__ RangeBounds source_bounds_lane(const unsigned* __restrict__ offsets,
+                                                          const TileGeom& g, int Tx, int Ty,
+                                                          int Tz, int lane) {
+  const int d = lane >> 3, cl = lane & 7;
+  const int single = (g.ntx == 1 ? 4 : 0) | (g.nty == 1 ? 2 : 0) | (g.ntz == 1 ? 1 : 0);
+  auto sel = [d](int c) { return (c & d) == d; };
+  int end = cl;
+  while (end < 7 && sel(end + 1)) ++end;
+  const unsigned Sx = (d & 4) ? (Tx ? Tx - 1 : g.ntx - 1) : Tx;
+  const unsigned Sy = (d & 2) ? (Ty ? Ty - 1 : g.nty - 1) : Ty;
+  const unsigned Sz = (d & 1) ? (Tz ? Tz - 1 : g.ntz - 1) : Tz;
+  const unsigned S8 = ((Sx * (unsigned)g.nty + Sy) * (unsigned)g.ntz + Sz) * 8u;
+  // every lane loads (S8 + end + 1 <= 8 ntiles, the last offset), so that
+  // the loads sit behind no branch whose end would have to wait for them
+  RangeBounds r;
+  r.bb = offsets[S8 + cl];
+  r.e = offsets[S8 + end + 1];
+  r.act = (d & single) == 0 && sel(cl) && (cl == 0 || !sel(cl - 1));
+  return r;
+}
+__device__ __forceinline__ void source_range_of(const RangeBounds& r, unsigned* b,
+                                                unsigned* len) {
+  *b = r.act ? r.bb : 0u;
+  *len = r.act && r.e > r.bb ? r.e - r.bb : 0u;
+}
 // Range of lane (d, cl) = (lane / 8, lane % 8) of tile_source_ranges: the
 // record range [b, b + len) of source tile T - d starting at class cl when a
 // run of selected classes starts there, else len = 0.
 __device__ __forceinline__ void source_range_lane(const unsigned* __restrict__ offsets,
                                                   const TileGeom& g, int Tx, int Ty, int Tz,
                                                   int lane, unsigned* b, unsigned* len) {
-  const int d = lane >> 3, cl = lane & 7;
-  const int single = (g.ntx == 1 ? 4 : 0) | (g.nty == 1 ? 2 : 0) | (g.ntz == 1 ? 1 : 0);
-  auto sel = [d](int c) { return (c & d) == d; };
-  *b = 0;
-  *len = 0;
-  if ((d & single) == 0 && sel(cl) && (cl == 0 || !sel(cl - 1))) {
-    int end = cl;
-    while (end < 7 && sel(end + 1)) ++end;
-    const unsigned Sx = (d & 4) ? (Tx ? Tx - 1 : g.ntx - 1) : Tx;
-    const unsigned Sy = (d & 2) ? (Ty ? Ty - 1 : g.nty - 1) : Ty;
-    const unsigned Sz = (d & 1) ? (Tz ? Tz - 1 : g.ntz - 1) : Tz;
-    const unsigned S8 = ((Sx * (unsigned)g.nty + Sy) * (unsigned)g.ntz + Sz) * 8u;
-    const unsigned bb = offsets[S8 + cl];
-    const unsigned e = offsets[S8 + end + 1];
-    *b = bb;
-    *len = e > bb ? e - bb : 0u;
-  }
+  source_range_of(source_bounds_lane(offsets, g, Tx, Ty, Tz, lane), b, len);
 }
 
 __device__ __forceinline__ void tile_source_ranges(const unsigned* __restrict__ offsets,
@@ -697,73 +714,86 @@ k_tile_paint(const float4* __restrict__ rec, const unsigned* __restrict__ offset
 // by the grid, so that the tiles in flight together stay neighbours: 2.6 GB
 // read per launch against 3.8 GB for consecutive tiles, same time) and
 // pipelines them: while wavefronts 0-7 transform and write tile
-// k (then clear their rows), wavefronts 8-15 find tile k + 1's source
-// ranges and load up to kR2cPre records each into registers.  The next
-// deposit starts with its records in hand; records beyond the prefetch are
-// read from the range table wavefront 8 leaves in LDS.  At 512^3 particles
-// on the clustered run: 2.90 ms per launch unpipelined; 2.53, 2.42-2.44,
-// 2.37 and 2.43 ms with 4, 8, 16 and 32 tiles per workgroup (3 records
-// prefetched per thread; 2 records: 2.46-2.48 at 8 tiles).
+// k (then clear their rows), wavefronts 8-15 load up to kR2cPre records
+// each of tile k + 1 into registers, whose source ranges they found one
+// tile earlier, and the bin offsets of tile k + 2, issued first, so that
+// the two loads of the chain overlap.  The next deposit starts with its
+// records in hand; records beyond the prefetch are read from the range
+// table wavefront 8 leaves in LDS (one table per tile in flight).  At
+// 512^3 particles on the clustered run: 2.90 ms per launch unpipelined;
+// 2.53, 2.42-2.44, 2.37 and 2.43 ms with 4, 8, 16 and 32 tiles per
+// workgroup and tile k + 1's offsets and records loaded in turn (3 records
+// prefetched per thread; 2 records: 2.46-2.48 at 8 tiles); 2.31-2.36 ms
+// against 2.40-2.44 with the offsets one tile further ahead.
 constexpr int kTileR2cBlock = 1024;
 constexpr int kR2cRowWaves = 8;
 constexpr unsigned kR2cLoaders = kTileR2cBlock - kR2cRowWaves * 64;
 constexpr int kR2cPre = 3;
 constexpr int kR2cTiles = 16;
 
-// Source ranges and prefetched records of tile tc, for one loader thread
-// (index lt < kR2cLoaders, lane `lane` of its wavefront).  Every loader
-// wavefront forms the tile's ranges itself (source_range_lane and a
-// wavefront scan; a record's range is found with scalar reads of the
-// range starts); `table` wavefronts also write them to rb / rpre / nranges
-// as tile_source_ranges does.  Returns the tile's record count.
-__device__ __forceinline__ unsigned r2c_prefetch(const float4* __restrict__ rec,
-                                                 const unsigned* __restrict__ offsets,
-                                                 const TileGeom& g, const TileCoord& tc,
-                                                 unsigned lt, int lane, bool table,
-                                                 unsigned* rb, unsigned* rpre, int* nranges,
-                                                 float4* pre) {
-  unsigned b, len;
-  source_range_lane(offsets, g, tc.Tx, tc.Ty, tc.Tz, lane, &b, &len);
+// A tile's source ranges as every loader wavefront holds them: this lane's
+// range start b and exclusive prefix, the lanes with a range, the total.
+struct R2cRanges {
+  unsigned b = 0, excl = 0, total = 0;
+  uint64_t nonempty = 0;
+};
+
+// The wavefront scan of the lanes' ranges (b, len); `table` wavefronts also
+// write them to rb / rpre / nranges as tile_source_ranges does.
+__device__ __forceinline__ R2cRanges r2c_scan(unsigned b, unsigned len, int lane, bool table,
+                                              unsigned* rb, unsigned* rpre, int* nranges) {
   unsigned incl = len;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const unsigned y = __shfl_up(incl, off);
     if (lane >= off) incl += y;
   }
-  const unsigned excl = incl - len;
-  const uint64_t nonempty = __ballot(len > 0);
-  const unsigned total = (unsigned)__shfl((int)incl, 63);
+  R2cRanges rs;
+  rs.b = b;
+  rs.excl = incl - len;
+  rs.nonempty = __ballot(len > 0);
+  rs.total = (unsigned)__builtin_amdgcn_readfirstlane(__shfl((int)incl, 63));
   if (table) {
-    const int k = __popcll(nonempty & ((1ull << lane) - 1ull));
+    const int k = __popcll(rs.nonempty & ((1ull << lane) - 1ull));
     if (len > 0) {
       rb[k] = b;
-      rpre[k] = excl;
+      rpre[k] = rs.excl;
     }
     if (lane == 63) {
-      *nranges = __popcll(nonempty);
-      rpre[__popcll(nonempty)] = incl;
+      *nranges = __popcll(rs.nonempty);
+      rpre[__popcll(rs.nonempty)] = incl;
     }
   }
+  return rs;
+}
+
+// Issues the loads of loader thread lt's first kR2cPre records of a tile
+// (index lt + i kR2cLoaders; a record's range is found with scalar reads
+// of the range starts).
+__device__ __forceinline__ void r2c_records(const float4* __restrict__ rec, const R2cRanges& rs,
+                                            unsigned lt, float4* pre) {
   unsigned src[kR2cPre];
 #pragma unroll
   for (int i = 0; i < kR2cPre; ++i) src[i] = 0u;
   // the ranges in order: record r lies in the last range whose start <= r
-  for (uint64_t m = nonempty; m; m &= m - 1ull) {
+  for (uint64_t m = rs.nonempty; m; m &= m - 1ull) {
     const int sl = __builtin_ctzll(m);
-    const unsigned p0 = (unsigned)__builtin_amdgcn_readlane((int)excl, sl);
-    const unsigned b0 = (unsigned)__builtin_amdgcn_readlane((int)b, sl);
+    const unsigned p0 = (unsigned)__builtin_amdgcn_readlane((int)rs.excl, sl);
+    const unsigned b0 = (unsigned)__builtin_amdgcn_readlane((int)rs.b, sl);
 #pragma unroll
     for (int i = 0; i < kR2cPre; ++i) {
       const unsigned r = lt + (unsigned)i * kR2cLoaders;
       if (r >= p0) src[i] = b0 + (r - p0);
     }
   }
+  // every thread loads: one past the tile's records reads record 0 (the
+  // record scratch holds at least one) and does not deposit it.  No load
+  // is behind a branch, so the caller's next wait can leave them in flight.
 #pragma unroll
   for (int i = 0; i < kR2cPre; ++i) {
     const unsigned r = lt + (unsigned)i * kR2cLoaders;
-    if (r < total) pre[i] = rec[src[i]];
+    pre[i] = rec[r < rs.total ? src[i] : 0u];
   }
-  return total;
 }
 
 template <int M>
@@ -782,8 +812,9 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
   ACC* acc = (ACC*)smem;
   __shared__ float2 stM[M];
   __shared__ float2 stN[M];
-  __shared__ unsigned rb[64], rpre[65];
-  __shared__ int nranges;
+  // range tables of tiles k and k + 1 (tile k's in rb[k & 1])
+  __shared__ unsigned rb[2][64], rpre[2][65];
+  __shared__ int nranges[2];
   for (int i = threadIdx.x; i < M; i += blockDim.x) {
     stM[i] = twM[i];
     stN[i] = twN[i];
@@ -793,6 +824,7 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
   lds_zero(smem, tcells * (int)sizeof(ACC));
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const bool loader = w >= kR2cRowWaves;
+  const bool table = w == kR2cRowWaves;
   const unsigned lt = threadIdx.x - (unsigned)(kR2cRowWaves * 64);
   // tile k of this workgroup: k * gridDim.x + its (XCD-remapped) index, so
   // that the tiles in flight at one time are neighbours, as in a one-shot
@@ -804,27 +836,37 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
   float4 pre[kR2cPre];
 #pragma unroll
   for (int i = 0; i < kR2cPre; ++i) pre[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  unsigned total = 0;
-  if (loader)
-    total = r2c_prefetch(rec, offsets, g, tile_coord(g, vb), lt, lane, w == kR2cRowWaves, rb,
-                         rpre, &nranges, pre);
-  __syncthreads();   // accumulators clear, twiddles and the first tile's table staged
+  R2cRanges cur, nxt;   // the ranges of tiles k and k + 1
+  if (loader) {
+    unsigned b, len;
+    const TileCoord t0 = tile_coord(g, vb);
+    source_range_lane(offsets, g, t0.Tx, t0.Ty, t0.Tz, lane, &b, &len);
+    cur = r2c_scan(b, len, lane, table, rb[0], rpre[0], &nranges[0]);
+    r2c_records(rec, cur, lt, pre);
+    if (kR2cTiles > 1) {
+      const TileCoord t1 = tile_coord(g, vb + vs);
+      source_range_lane(offsets, g, t1.Tx, t1.Ty, t1.Tz, lane, &b, &len);
+      nxt = r2c_scan(b, len, lane, table, rb[1], rpre[1], &nranges[1]);
+    }
+  }
+  __syncthreads();   // accumulators clear, twiddles and the first two tables staged
 #pragma unroll 1
   for (int k = 0; k < kR2cTiles; ++k) {
+    const int tb = k & 1;
     const TileCoord tc = tile_coord(g, vb + (unsigned)k * vs);
     const int x0 = tc.Tx * g.tx, y0 = tc.Ty * g.ty, z0 = tc.Tz * g.tz;
     if (loader) {
 #pragma unroll
       for (int i = 0; i < kR2cPre; ++i)
-        if (lt + (unsigned)i * kR2cLoaders < total)
+        if (lt + (unsigned)i * kR2cLoaders < cur.total)
           deposit_record<ACC>(acc, pre[i], g, x0, y0, z0, f0);
     }
     // records beyond the prefetch, located through the LDS range table
-    const unsigned tot = rpre[nranges];
+    const unsigned tot = rpre[tb][nranges[tb]];
     int r = 0;
     for (unsigned v = kR2cLoaders * kR2cPre + threadIdx.x; v < tot; v += blockDim.x) {
-      while (v >= rpre[r + 1]) ++r;
-      deposit_record<ACC>(acc, rec[rb[r] + (v - rpre[r])], g, x0, y0, z0, f0);
+      while (v >= rpre[tb][r + 1]) ++r;
+      deposit_record<ACC>(acc, rec[rb[tb][r] + (v - rpre[tb][r])], g, x0, y0, z0, f0);
     }
     __syncthreads();   // the tile's cells are complete
     if (!loader) {
@@ -866,10 +908,22 @@ k_tile_paint_r2c(const float4* __restrict__ rec, const unsigned* __restrict__ of
           a4[t + i * 64] = make_uint4(0u, 0u, 0u, 0u);
       }
     } else if (k + 1 < kR2cTiles) {
-      total = r2c_prefetch(rec, offsets, g, tile_coord(g, vb + (unsigned)(k + 1) * vs), lt, lane,
-                           w == kR2cRowWaves, rb, rpre, &nranges, pre);
+      // tile k + 2's bin offsets are loaded first, then tile k + 1's records
+      // (its ranges are known), then the offsets are used: the next deposit
+      // waits on one load latency, not on two in a row.  Tile k + 2's table
+      // replaces tile k's, which this tile's deposit finished with above.
+      const bool two = k + 2 < kR2cTiles;
+      const TileCoord t2 = tile_coord(g, vb + (unsigned)(two ? k + 2 : k + 1) * vs);
+      const RangeBounds rk2 = source_bounds_lane(offsets, g, t2.Tx, t2.Ty, t2.Tz, lane);
+      r2c_records(rec, nxt, lt, pre);
+      cur = nxt;
+      if (two) {
+        unsigned b, len;
+        source_range_of(rk2, &b, &len);
+        nxt = r2c_scan(b, len, lane, table, rb[tb], rpre[tb], &nranges[tb]);
+      }
     }
-    __syncthreads();   // rows written and cleared, the next tile's table staged
+    __syncthreads();   // rows written and cleared, tile k + 2's table staged
   }
 }
 
