@@ -1213,11 +1213,13 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
     const unsigned col = ((unsigned)Cx * nty + (unsigned)Cy) * ntz;
     const unsigned pbeg = offsets[col * 8u], pend = offsets[(col + ntz) * 8u];
     // this thread's first two records of the column, in flight during the
-    // transform
+    // transform.  The loads are unconditional (record 0, which any record
+    // scratch holds, past the column's end; not read out): a load behind a
+    // branch is waited for where the branch ends, with every load before
+    // it, the row prefetched for this unit's transform included.
     const unsigned j0 = pbeg + threadIdx.x, j1 = j0 + blockDim.x;
-    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
-    if (j0 < pend) r0 = rec[j0];
-    if (j1 < pend) r1 = rec[j1];
+    const float4 r0 = rec[j0 < pend ? j0 : 0u];
+    const float4 r1 = rec[j1 < pend ? j1 : 0u];
     const int nrows = c == 0 ? kColRows : kColNew;
     int r = w0;
     if (r < nrows) {   // the row loaded during the last unit
