@@ -66,7 +66,8 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
   using L = RowsLayout<P>;
   __shared__ float2 s[L::rows(N) * P];
   __shared__ float2 stw[N];
-  stage_twiddles<N>(stw, tw);
+  TwiddleStage<N, T * P> tws;
+  tws.load(tw);   // stored after the pencil's loads; fft_regs' first barrier orders it
   src += blockIdx.y * bstride;   // mesh of this block's box (batched launches)
   dst += blockIdx.y * bstride;
   const int tid = threadIdx.x;
@@ -79,15 +80,19 @@ k_fft_pencil(const float2* src, float2* dst, int ncols, int64_t elem_stride,
   const int base = o * (int)outer_stride + c;
   const int es = (int)elem_stride;
   float2 x[N / T];
+  // Loads are unconditional: a column past ncols reads the row padding
+  // (rows hold grid_cpitch(n) >= nchunks P elements) and is zeroed after, so
+  // that no branch end waits for the loads.
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
     const int j = t + i * T;
-    float2 v = make_float2(0.f, 0.f);
-    if (valid) v = src[base + j * es];
+    float2 v = src[base + j * es];
+    if (!valid) v = make_float2(0.f, 0.f);
     if constexpr (GREEN) v = green_mul(v, j, o, c, g);
     if constexpr (INV) v.y = -v.y;
     x[i] = v;
   }
+  tws.store(stw);
   fft_regs<N, L>(x, s + p, t, stw);
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
@@ -111,7 +116,8 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, int ncols,
   using L = RowsLayout<P>;
   __shared__ float2 s[L::rows(N) * P];
   __shared__ float2 stw[N];
-  stage_twiddles<N>(stw, tw);
+  TwiddleStage<N, T * P> tws;
+  tws.load(tw);   // stored after the pencil's loads; fft_regs' first barrier orders it
   const int64_t boff = blockIdx.y * bstride;   // mesh of this block's box (batched launches)
   src += boff;
   dst0 += boff;
@@ -125,11 +131,14 @@ k_fft_x_green3(const float2* src, float2* dst0, float2* dst1, int ncols,
   const int base = o * (int)outer_stride + c;
   const int es = (int)elem_stride;
   float2 dk[N / T];
+  // unconditional loads, as in k_fft_pencil
 #pragma unroll
   for (int i = 0; i < N / T; ++i) {
     const int j = t + i * T;
-    dk[i] = valid ? src[base + j * es] : make_float2(0.f, 0.f);
+    const float2 v = src[base + j * es];
+    dk[i] = valid ? v : make_float2(0.f, 0.f);
   }
+  tws.store(stw);
   fft_regs<N, L>(dk, s + p, t, stw);
   // Along an x pencil ky and kz are constant, so the y and z force spectra
   // are ky and kz times one transform G = IFFT_x(i f1/|k|^2 delta_k): two
@@ -183,7 +192,8 @@ k_fft_y_inv_dual(float2* g1, float2* dst2, int ncols, int64_t elem_stride, int64
   using L = RowsLayout<P>;
   __shared__ float2 s[L::rows(N) * P];
   __shared__ float2 stw[N];
-  stage_twiddles<N>(stw, tw);
+  TwiddleStage<N, T * P> tws;
+  tws.load(tw);   // stored after the pencil's loads; fft_regs' first barrier orders it
   g1 += blockIdx.y * bstride;
   dst2 += blockIdx.y * bstride;
   const int tid = threadIdx.x;
@@ -196,9 +206,13 @@ k_fft_y_inv_dual(float2* g1, float2* dst2, int ncols, int64_t elem_stride, int64
   const int es = (int)elem_stride;
   const float kz = (float)c;
   float2 gk[N / T];
+  // unconditional loads, as in k_fft_pencil
 #pragma unroll
-  for (int i = 0; i < N / T; ++i)
-    gk[i] = valid ? g1[base + (t + i * T) * es] : make_float2(0.f, 0.f);
+  for (int i = 0; i < N / T; ++i) {
+    const float2 v = g1[base + (t + i * T) * es];
+    gk[i] = valid ? v : make_float2(0.f, 0.f);
+  }
+  tws.store(stw);
 #pragma unroll 1
   for (int pass = 0; pass < 2; ++pass) {
     int tl = t, pl = p;

@@ -315,5 +315,31 @@ __device__ __forceinline__ void stage_twiddles(float2* stw, const float2* __rest
   for (int i = threadIdx.x; i < N; i += blockDim.x) stw[i] = tw[i];
 }
 
+// The same copy in two halves for blocks of THREADS threads: load() before
+// the caller's own global loads, store() after them, so that the table's
+// latency overlaps theirs (the loop above waits for each table load before
+// any later load is issued).  The loads are unconditional, so that no
+// branch end waits for them.  A workgroup barrier must follow store()
+// before the table is read.
+template <int N, int THREADS>
+struct TwiddleStage {
+  static constexpr int K = (N + THREADS - 1) / THREADS;
+  float2 v[K];
+  __device__ __forceinline__ void load(const float2* __restrict__ tw) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = (int)threadIdx.x + k * THREADS;
+      v[k] = tw[i < N ? i : N - 1];
+    }
+  }
+  __device__ __forceinline__ void store(float2* stw) const {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = (int)threadIdx.x + k * THREADS;
+      if (i < N) stw[i] = v[k];
+    }
+  }
+};
+
 }  // namespace fftdev
 }  // namespace fs2
