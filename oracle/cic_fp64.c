@@ -2,7 +2,8 @@
  * cic_fp64.c -- FP64 CIC mass assignment and readout (test oracle only).
  *
  * Test infrastructure: only tests/, the smoke check and the bench's CPU
- * baseline load this library, as the checker; the product path never does.
+ * baseline load this library, to compare against; the product path never
+ * does.
  *
  * The same arithmetic as oracle/cic.py (wdtype float64), in C so that the
  * large-mesh parity tests (16.8M and 134M particles) finish in seconds:
