@@ -1,7 +1,7 @@
 """CPU oracle for the PM + COLA hot path -- TEST INFRASTRUCTURE ONLY.
 
 This package restates, in FP64 numpy/scipy, the algorithms of the fs2
-reference that the HIP path replaces.  It is the checker: only ``tests/``,
+reference that the HIP path replaces, to check it against: only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
 it, and never as the thing measured or shipped.  ``fs2_amd`` never imports it.
 
