@@ -348,13 +348,16 @@ k_fft_z_r2c(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
   const int t = tid % T, p = tid / T;
   const int64_t row = (int64_t)blockIdx.x * P + p;
   const bool valid = row < nrows;
-  float2* r = data + row * cpitch;
+  // a row past the end loads the last row (and stores nothing), so that
+  // the loads are unconditional and no branch end waits for them
+  float2* r = data + (valid ? row : nrows - 1) * cpitch;
   float2* sp = s + p * LS;
   float2 x[M / T];
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int m = t + i * T;
-    x[i] = valid ? r[m] : make_float2(0.f, 0.f);
+    const float2 v = r[m];
+    x[i] = valid ? v : make_float2(0.f, 0.f);
   }
   using S = RowSync<M>;
   fft_regs<M, L, S>(x, sp, t, twM);
@@ -395,13 +398,16 @@ k_fft_z_r2c_lt(float2* data, int64_t nrows, int cpitch, const float2* __restrict
   const int t = tid % T, p = tid / T;
   const int64_t row = (int64_t)blockIdx.x * P + p;
   const bool valid = row < nrows;
-  float2* r = data + row * cpitch;
+  // a row past the end loads the last row (and stores nothing), so that
+  // the loads are unconditional and no branch end waits for them
+  float2* r = data + (valid ? row : nrows - 1) * cpitch;
   float2* sp = s + p * LS;
   float2 x[M / T];
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int m = t + i * T;
-    x[i] = valid ? r[m] : make_float2(0.f, 0.f);
+    const float2 v = r[m];
+    x[i] = valid ? v : make_float2(0.f, 0.f);
   }
   using S = RowSync<M>;
   __syncthreads();   // the twiddles are staged
@@ -437,16 +443,23 @@ k_fft_z_c2r(float2* data, int64_t nrows, int cpitch, const float2* __restrict__ 
   const int t = tid % T, p = tid / T;
   const int64_t row = (int64_t)blockIdx.x * P + p;
   const bool valid = row < nrows;
-  float2* r = data + row * cpitch;
+  // unconditional loads, as in k_fft_z_r2c
+  float2* r = data + (valid ? row : nrows - 1) * cpitch;
   float2* sp = s + p * LS;
   // X[k] and X[M - k] straight from the row (the mirrored loads hit the
   // lines the direct ones bring in)
-  float2 z[M / T];
+  float2 z[M / T], zb[M / T];
+#pragma unroll
+  for (int i = 0; i < M / T; ++i) {   // every load issued before any use
+    const int k = t + i * T;
+    z[i] = r[k];
+    zb[i] = r[M - k];
+  }
 #pragma unroll
   for (int i = 0; i < M / T; ++i) {
     const int k = t + i * T;
-    const float2 A = valid ? r[k] : make_float2(0.f, 0.f);
-    const float2 B = valid ? r[M - k] : make_float2(0.f, 0.f);
+    const float2 A = valid ? z[i] : make_float2(0.f, 0.f);
+    const float2 B = valid ? zb[i] : make_float2(0.f, 0.f);
     z[i] = c2r_merge(A, B, twN[k], k == 0);
   }
   fft_regs<M, L, RowSync<M>>(z, sp, t, twM);
