@@ -1212,14 +1212,15 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
     const int y0 = Cy * kColY;
     const unsigned col = ((unsigned)Cx * nty + (unsigned)Cy) * ntz;
     const unsigned pbeg = offsets[col * 8u], pend = offsets[(col + ntz) * 8u];
-    // this thread's first two records of the column, in flight during the
+    // this thread's first three records of the column, in flight during the
     // transform.  The loads are unconditional (record 0, which any record
     // scratch holds, past the column's end; not read out): a load behind a
     // branch is waited for where the branch ends, with every load before
     // it, the row prefetched for this unit's transform included.
-    const unsigned j0 = pbeg + threadIdx.x, j1 = j0 + blockDim.x;
+    const unsigned j0 = pbeg + threadIdx.x, j1 = j0 + blockDim.x, j2 = j1 + blockDim.x;
     const float4 r0 = rec[j0 < pend ? j0 : 0u];
     const float4 r1 = rec[j1 < pend ? j1 : 0u];
+    const float4 r2 = rec[j2 < pend ? j2 : 0u];
     const int nrows = c == 0 ? kColRows : kColNew;
     int r = w0;
     if (r < nrows) {   // the row loaded during the last unit
@@ -1247,7 +1248,8 @@ k_column_readout(const float2* __restrict__ s0, const float2* __restrict__ s1,
     const int ring0 = (kColY * c) % kColHY;
     if (j0 < pend) column_readout_one<RS>(h, r0, dx_inv, n, x0, y0, ring0, fa);
     if (j1 < pend) column_readout_one<RS>(h, r1, dx_inv, n, x0, y0, ring0, fa);
-    for (unsigned j = j1 + blockDim.x; j < pend; j += blockDim.x)
+    if (j2 < pend) column_readout_one<RS>(h, r2, dx_inv, n, x0, y0, ring0, fa);
+    for (unsigned j = j2 + blockDim.x; j < pend; j += blockDim.x)
       column_readout_one<RS>(h, rec[j], dx_inv, n, x0, y0, ring0, fa);
     __syncthreads();   // the rows are read before the next unit overwrites them
   }
