@@ -412,6 +412,39 @@ def test_fused_path_with_empty_and_sparse_sets(fs):
     assert np.array_equal(p.force, f_fused), np.abs(p.force - f_fused).max()
 
 
+@pytest.mark.timeout(300)
+def test_paint_variants_agree_at_1024(fs):
+    """At 1024^3 the three paints take different paths: sorted (the
+    default) fuses its tile kernel with the z r2c and is read out by the
+    column kernel; the LDS-atomic tile paint writes the density first and
+    is still read out by the column kernel; the global-atomic paint runs
+    the unfused path (z r2c, z c2r passes, gather readout).  On one
+    clustered set their forces agree to float-atomic rounding."""
+    _announce("paint variants at 1024^3, 64^3 clustered particles")
+    n_p, nc, L = 64, 1024, 1024.0
+    x = np.mod(_lattice_positions(n_p, L, 3.0, seed=29), L)
+    x[:20000] = np.mod(300.0 + 2.0 * np.random.default_rng(30).standard_normal((20000, 3)), L)
+    fs.pm.init(nc, nc / n_p, L)
+    forces = {}
+    try:
+        for variant in ("sorted", "atomic", "global"):
+            fs.pm.set_paint_variant(variant)
+            p = fs.Particles(n_p**3, L)
+            p.append(x)
+            p.set_times(0.5, 0.5, 0.0)
+            fs.pm.force(p)
+            forces[variant] = p.force.astype(np.float64)
+    finally:
+        fs.pm.set_paint_variant("sorted")
+    ref = forces["sorted"]
+    scale = np.sqrt(np.mean(ref**2))
+    assert np.isfinite(ref).all() and scale > 0
+    for variant in ("atomic", "global"):
+        rel = float(np.sqrt(np.mean((forces[variant] - ref) ** 2)) / scale)
+        print(f"[case] paint {variant} vs sorted at 1024^3: force rel RMS {rel:.2e}")
+        assert rel < 1e-5, (variant, rel)
+
+
 def test_mem_named_allocation(fs):
     """fs.mem.Mem (py/fs/mem.py): sizes rounded up to 16 bytes (mem.cpp:48-50),
     nothing allocated for size 0, MemoryError when the device cannot hold it."""
